@@ -1,0 +1,215 @@
+"""
+CPU oracle for the pairwise gravity path -- TEST INFRASTRUCTURE ONLY.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py`` may import this package; it is the checker, never the product.
+The ``harmonica_amd`` package does not import it and has no CPU fallback.
+
+The functions below are thin ctypes wrappers around ``oracle.c`` (a C99 +
+OpenMP restatement of the reference's Numba loops and of the closed forms
+they call; see the citations in that file) that reproduce the reference's
+Python-level post-processing: sign flips and unit conversions of
+``src/harmonica/_forward/prisms/gravity.py:228-235``,
+``src/harmonica/_forward/prisms/layer.py:425-433`` and
+``src/harmonica/_forward/point.py:253-260``.
+"""
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+#: Field names in the reference's FIELDS order (gravity.py:37-48).
+FIELDS = ("potential", "g_e", "g_n", "g_z", "g_ee", "g_nn", "g_zz", "g_en", "g_ez", "g_nz")
+FIELD_ID = {f: i for i, f in enumerate(FIELDS)}
+GRAVITATIONAL_CONST = 6.6743e-11
+
+_lib = None
+
+
+def build():
+    """Compile liboracle.so from oracle.c with the Makefile beside it."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    """Load (building first if absent) the oracle shared library."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        dp = ctypes.POINTER(ctypes.c_double)
+        i64 = ctypes.c_int64
+        ci = ctypes.c_int
+        cd = ctypes.c_double
+        L.or_prism_gravity.argtypes = [dp, dp, dp, i64, dp, dp, i64, ci, dp, ci]
+        L.or_prism_any_singular.argtypes = [dp, dp, dp, i64, dp, i64, ci]
+        L.or_prism_any_singular.restype = ci
+        L.or_prism_layer.argtypes = [dp, dp, dp, i64, dp, i64, dp, i64, dp, dp, dp, cd, ci, dp, ci]
+        L.or_prism_pair.argtypes = [ci, cd, cd, cd, dp, cd]
+        L.or_prism_pair.restype = cd
+        L.or_prism_dgu_dtop.argtypes = [cd, cd, cd, dp, cd]
+        L.or_prism_dgu_dtop.restype = cd
+        L.or_point_pair.argtypes = [ci, cd, cd, cd, cd, cd, cd, cd]
+        L.or_point_pair.restype = cd
+        L.or_point_cartesian.argtypes = [dp, dp, dp, i64, dp, dp, dp, dp, i64, ci, dp, ci]
+        L.or_point_spherical.argtypes = [dp, dp, dp, i64, dp, dp, dp, dp, i64, ci, dp, ci]
+        L.or_eqs_jacobian.argtypes = [dp, dp, dp, i64, dp, dp, dp, i64, dp, ci]
+        L.or_eqs_predict.argtypes = [dp, dp, dp, i64, dp, dp, dp, dp, i64, dp, ci]
+        L.or_num_threads.restype = ci
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _f64(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+
+
+def _units(field, result):
+    """Sign flip and unit conversion (gravity.py:228-235, point.py:253-260)."""
+    if field in ("g_z", "g_ez", "g_nz", "g_ze", "g_zn"):
+        result *= -1
+    if field in ("g_e", "g_n", "g_z"):
+        result *= 1e5
+    if field in ("g_ee", "g_nn", "g_zz", "g_en", "g_ez", "g_nz", "g_ne", "g_ze", "g_zn"):
+        result *= 1e9
+    return result
+
+
+def _coords(coordinates):
+    cast = np.broadcast(*coordinates[:3])
+    arrs = tuple(_f64(np.broadcast_to(c, cast.shape).ravel()) for c in coordinates[:3])
+    return cast.shape, arrs
+
+
+def prism_gravity(coordinates, prisms, density, field, n_threads=0):
+    """Oracle of ``harmonica.prism_gravity`` (gravity.py:51-236) without the
+    input checks; null prisms are kept (they contribute exactly zero)."""
+    shape, (e, n, u) = _coords(coordinates)
+    prisms = _f64(np.atleast_2d(prisms))
+    density = _f64(np.atleast_1d(density).ravel())
+    out = np.zeros(e.size)
+    lib().or_prism_gravity(_p(e), _p(n), _p(u), e.size, _p(prisms), _p(density),
+                           prisms.shape[0], FIELD_ID[field], _p(out), n_threads)
+    return _units(field, out).reshape(shape)
+
+
+def prism_any_singular(coordinates, prisms, field):
+    """Oracle of ``_check_singular_points`` (gravity.py:239-449)."""
+    _, (e, n, u) = _coords(coordinates)
+    prisms = _f64(np.atleast_2d(prisms))
+    return bool(lib().or_prism_any_singular(_p(e), _p(n), _p(u), e.size, _p(prisms),
+                                            prisms.shape[0], FIELD_ID[field]))
+
+
+def prism_pair(field, point, prism, density):
+    """One prism at one point, SI units, before sign flip/units."""
+    prism = _f64(prism)
+    return lib().or_prism_pair(FIELD_ID[field], float(point[0]), float(point[1]),
+                               float(point[2]), _p(prism), float(density))
+
+
+def prism_dgu_dtop(point, prism, density):
+    """d(g_u SI)/d(top) of one prism at one point."""
+    prism = _f64(prism)
+    return lib().or_prism_dgu_dtop(float(point[0]), float(point[1]), float(point[2]),
+                                   _p(prism), float(density))
+
+
+def prism_layer_gravity(coordinates, easting, northing, bottom, top, density, field,
+                        thickness_threshold=None, n_threads=0):
+    """Oracle of ``DatasetAccessorPrismLayer.gravity`` (layer.py:314-433)."""
+    shape, (e, n, u) = _coords(coordinates)
+    ce, cn = _f64(easting), _f64(northing)
+    b, t, d = _f64(bottom), _f64(top), _f64(density)
+    thr = 0.0 if thickness_threshold is None else float(thickness_threshold)
+    out = np.zeros(e.size)
+    lib().or_prism_layer(_p(e), _p(n), _p(u), e.size, _p(ce), ce.size, _p(cn), cn.size,
+                         _p(b), _p(t), _p(d), thr, FIELD_ID[field], _p(out), n_threads)
+    return _units(field, out).reshape(shape)
+
+
+_ALIASES = {"g_ne": "g_en", "g_ze": "g_ez", "g_zn": "g_nz"}
+
+
+def point_gravity(coordinates, points, masses, field, coordinate_system="cartesian",
+                  n_threads=0):
+    """Oracle of ``harmonica.point_gravity`` (point.py:30-261)."""
+    shape, (e, n, u) = _coords(coordinates)
+    pe, pn, pu = (_f64(np.atleast_1d(p).ravel()) for p in points[:3])
+    m = _f64(np.atleast_1d(masses).ravel())
+    out = np.zeros(e.size)
+    fid = FIELD_ID[_ALIASES.get(field, field)]
+    if coordinate_system == "cartesian":
+        lib().or_point_cartesian(_p(e), _p(n), _p(u), e.size, _p(pe), _p(pn), _p(pu),
+                                 _p(m), m.size, fid, _p(out), n_threads)
+    else:
+        lib().or_point_spherical(_p(e), _p(n), _p(u), e.size, _p(pe), _p(pn), _p(pu),
+                                 _p(m), m.size, fid, _p(out), n_threads)
+    return _units(field, out).reshape(shape)
+
+
+def point_pair(field, point, source, mass):
+    return lib().or_point_pair(FIELD_ID[field], *(float(v) for v in point),
+                               *(float(v) for v in source), float(mass))
+
+
+def eqs_jacobian(coordinates, points, n_threads=0):
+    """Oracle of ``EquivalentSources.jacobian`` (cartesian.py:385-415)."""
+    e, n, u = (_f64(np.atleast_1d(c).ravel()) for c in coordinates[:3])
+    pe, pn, pu = (_f64(np.atleast_1d(c).ravel()) for c in points[:3])
+    jac = np.zeros((e.size, pe.size))
+    lib().or_eqs_jacobian(_p(e), _p(n), _p(u), e.size, _p(pe), _p(pn), _p(pu), pe.size,
+                          _p(jac), n_threads)
+    return jac
+
+
+def eqs_predict(coordinates, points, coefs, n_threads=0):
+    """Oracle of ``EquivalentSources.predict``'s loop (utils.py:77-97)."""
+    shape, (e, n, u) = _coords(coordinates)
+    pe, pn, pu = (_f64(np.atleast_1d(c).ravel()) for c in points[:3])
+    c = _f64(coefs)
+    out = np.zeros(e.size)
+    lib().or_eqs_predict(_p(e), _p(n), _p(u), e.size, _p(pe), _p(pn), _p(pu), _p(c),
+                         pe.size, _p(out), n_threads)
+    return out.reshape(shape)
+
+
+def eqs_fit_coefs(jacobian, data, weights=None, damping=None):
+    """Restatement of verde's ``least_squares`` as called at cartesian.py:280:
+    columns scaled by their population standard deviation (zero -> 1), then
+    weighted ridge normal equations (damping None -> plain least squares),
+    then unscaled.  Pinned in the tests against scikit-learn's
+    StandardScaler(with_mean=False) + Ridge(fit_intercept=False)."""
+    J = np.asarray(jacobian, dtype=np.float64)
+    d = np.asarray(data, dtype=np.float64).ravel()
+    scale = J.std(axis=0)
+    scale[scale == 0] = 1.0
+    Js = J / scale
+    if weights is None:
+        w = np.ones_like(d)
+    else:
+        w = np.asarray(weights, dtype=np.float64).ravel()
+    if damping is None:
+        sw = np.sqrt(w)
+        sol = np.linalg.lstsq(Js * sw[:, None], d * sw, rcond=None)[0]
+    else:
+        A = Js.T @ (Js * w[:, None]) + damping * np.eye(J.shape[1])
+        b = Js.T @ (w * d)
+        import scipy.linalg
+
+        sol = scipy.linalg.solve(A, b, assume_a="pos")
+    return sol / scale
+
+
+def num_threads():
+    return lib().or_num_threads()
