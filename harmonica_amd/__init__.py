@@ -1,0 +1,30 @@
+"""
+harmonica_amd -- MI355X (gfx950) implementation of harmonica's pairwise
+gravity forward-modelling path.
+
+Drop-in functions with the reference's Python API (fatiando/harmonica):
+
+* ``prism_gravity`` (all ten fields, singular-point handling),
+* ``prism_layer`` and its ``.prism_layer`` accessor (``.gravity``),
+* ``point_gravity`` (Cartesian and spherical),
+* ``EquivalentSources`` (jacobian / predict / fit / grid / profile),
+
+computed by hand-written HIP kernels in libhmx.so (C-ABI in include/hmx.h),
+called through ctypes, with PyTorch-ROCm used only for device buffers and
+streams.  There is no CPU fallback: without the library or a GPU the
+functions raise.
+"""
+
+from ._forward import point_gravity, prism_gravity
+from ._layer import DatasetAccessorPrismLayer, prism_layer
+from .constants import GRAVITATIONAL_CONST
+
+__all__ = [
+    "GRAVITATIONAL_CONST",
+    "DatasetAccessorPrismLayer",
+    "point_gravity",
+    "prism_gravity",
+    "prism_layer",
+]
+
+__version__ = "0.1.0"

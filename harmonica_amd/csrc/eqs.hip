@@ -1,0 +1,172 @@
+// Equivalent-source jacobian and predict kernels (Green's function 1 / r).
+//
+// jacobian: one workgroup row-block x column-tile; consecutive lanes write
+// consecutive columns of the row-major (n_data, ld) matrix (coalesced
+// stores; the kernel is HBM-write bound).
+// predict: 256 observation points per workgroup, sources and coefficients
+// staged through LDS, FP64 accumulation (also for FP32 data).
+#include "hmx_internal.h"
+
+namespace hmx {
+namespace {
+
+template <class T>
+__global__ __launch_bounds__(256) void eqs_jacobian_kernel(
+    const T *__restrict__ e, const T *__restrict__ n, const T *__restrict__ u, int64_t n_data,
+    const T *__restrict__ pe, const T *__restrict__ pn, const T *__restrict__ pu, int64_t n_src,
+    T *__restrict__ jac, int64_t ld) {
+    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (j >= n_src) return;
+    const T sj = pe[j], nj = pn[j], uj = pu[j];
+    for (int64_t i = blockIdx.y; i < n_data; i += gridDim.y) {
+        const T de = e[i] - sj, dn = n[i] - nj, du = u[i] - uj;
+        jac[i * ld + j] = T(1) / sqrt(de * de + dn * dn + du * du);
+    }
+}
+
+template <class T>
+struct alignas(16) StagedSource {
+    T e, n, u, c;
+};
+
+template <class T>
+__global__ __launch_bounds__(kObsPerBlock) void eqs_predict_kernel(
+    const T *__restrict__ e, const T *__restrict__ n, const T *__restrict__ u, int n_obs,
+    const T *__restrict__ pe, const T *__restrict__ pn, const T *__restrict__ pu,
+    const T *__restrict__ coefs, int64_t n_src, int64_t chunk_len, double *__restrict__ partial) {
+    __shared__ StagedSource<T> tile[kTile];
+    const int i = blockIdx.x * kObsPerBlock + threadIdx.x;
+    const bool valid = i < n_obs;
+    const int ii = valid ? i : n_obs - 1;
+    const T oe = e[ii], on = n[ii], ou = u[ii];
+    double acc = 0.0;
+    const int64_t c0 = (int64_t)blockIdx.y * chunk_len;
+    const int64_t c1 = c0 + chunk_len < n_src ? c0 + chunk_len : n_src;
+    for (int64_t t0 = c0; t0 < c1; t0 += kTile) {
+        const int cnt = (int)(c1 - t0 < kTile ? c1 - t0 : kTile);
+        __syncthreads();
+        for (int q = threadIdx.x; q < cnt; q += blockDim.x) {
+            StagedSource<T> s;
+            s.e = pe[t0 + q];
+            s.n = pn[t0 + q];
+            s.u = pu[t0 + q];
+            s.c = coefs[t0 + q];
+            tile[q] = s;
+        }
+        __syncthreads();
+        for (int p = 0; p < cnt; ++p) {
+            const StagedSource<T> s = tile[p];
+            const T de = oe - s.e, dn = on - s.n, du = ou - s.u;
+            acc += (double)(s.c * (T(1) / sqrt(de * de + dn * dn + du * du)));
+        }
+    }
+    if (valid) partial[(int64_t)blockIdx.y * n_obs + i] = acc;
+}
+
+template <class T>
+__global__ void eqs_predict_reduce(const double *__restrict__ partial, int64_t chunks,
+                                   int64_t n_obs, T *__restrict__ out) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_obs) return;
+    double sum = 0.0;
+    for (int64_t c = 0; c < chunks; ++c) sum += partial[c * n_obs + i];
+    out[i] = (T)sum;
+}
+
+template <class T>
+int jacobian_typed(const void *e, const void *n, const void *u, int64_t n_data, const void *pe,
+                   const void *pn, const void *pu, int64_t n_src, void *jac, int64_t ld,
+                   hipStream_t s) {
+    unsigned gy = (unsigned)(n_data < 8192 ? n_data : 8192);
+    dim3 grid((unsigned)((n_src + 255) / 256), gy);
+    hipLaunchKernelGGL(eqs_jacobian_kernel<T>, grid, dim3(256), 0, s, (const T *)e, (const T *)n,
+                       (const T *)u, n_data, (const T *)pe, (const T *)pn, (const T *)pu, n_src,
+                       (T *)jac, ld);
+    HMX_LAUNCHED("eqs_jacobian_kernel");
+    return HMX_OK;
+}
+
+template <class T>
+int predict_typed(const void *e, const void *n, const void *u, int64_t n_obs, const void *pe,
+                  const void *pn, const void *pu, const void *coefs, int64_t n_src, void *out,
+                  double *partial, int64_t chunks, hipStream_t s) {
+    const int64_t chunk_len = (n_src + chunks - 1) / chunks;
+    if (n_src > 0) {
+        dim3 grid((unsigned)((n_obs + kObsPerBlock - 1) / kObsPerBlock), (unsigned)chunks);
+        hipLaunchKernelGGL(eqs_predict_kernel<T>, grid, dim3(kObsPerBlock), 0, s, (const T *)e,
+                           (const T *)n, (const T *)u, (int)n_obs, (const T *)pe, (const T *)pn,
+                           (const T *)pu, (const T *)coefs, n_src, chunk_len, partial);
+        HMX_LAUNCHED("eqs_predict_kernel");
+    }
+    hipLaunchKernelGGL(eqs_predict_reduce<T>, dim3((unsigned)((n_obs + 255) / 256)), dim3(256), 0,
+                       s, partial, n_src > 0 ? chunks : 0, n_obs, (T *)out);
+    HMX_LAUNCHED("eqs_predict_reduce");
+    return HMX_OK;
+}
+
+}  // namespace
+}  // namespace hmx
+
+extern "C" {
+
+int hmx_eqs_jacobian(const void *easting, const void *northing, const void *upward,
+                     int64_t n_data, const void *p_easting, const void *p_northing,
+                     const void *p_upward, int64_t n_src, int dtype, void *jac, int64_t ld,
+                     void *stream) {
+    using namespace hmx;
+    clear_error();
+    if (n_data < 0 || n_src < 0) return fail(HMX_ERR_ARG, "negative size");
+    if (ld < n_src) return fail(HMX_ERR_ARG, "ld (%lld) < n_src (%lld)", (long long)ld, (long long)n_src);
+    if (n_data == 0 || n_src == 0) return HMX_OK;
+    if (!easting || !northing || !upward || !p_easting || !p_northing || !p_upward || !jac)
+        return fail(HMX_ERR_ARG, "NULL array");
+    hipStream_t s = (hipStream_t)stream;
+    if (dtype == HMX_F64)
+        return jacobian_typed<double>(easting, northing, upward, n_data, p_easting, p_northing,
+                                      p_upward, n_src, jac, ld, s);
+    if (dtype == HMX_F32)
+        return jacobian_typed<float>(easting, northing, upward, n_data, p_easting, p_northing,
+                                     p_upward, n_src, jac, ld, s);
+    return fail(HMX_ERR_ARG, "invalid dtype %d", dtype);
+}
+
+int hmx_eqs_predict_workspace_bytes(int64_t n_obs, int64_t n_src, size_t *bytes) {
+    using namespace hmx;
+    if (!bytes) return fail(HMX_ERR_ARG, "bytes is NULL");
+    if (n_obs < 0 || n_src < 0) return fail(HMX_ERR_ARG, "negative size");
+    int cu = device_cu_count();
+    if (cu < 0) return cu;
+    *bytes = (size_t)forward_chunks(n_obs, n_src, cu) * (size_t)n_obs * sizeof(double);
+    return HMX_OK;
+}
+
+int hmx_eqs_predict(const void *easting, const void *northing, const void *upward,
+                    int64_t n_obs, const void *p_easting, const void *p_northing,
+                    const void *p_upward, const void *coefs, int64_t n_src, int dtype,
+                    void *out, void *workspace, size_t workspace_bytes, void *stream) {
+    using namespace hmx;
+    clear_error();
+    if (n_obs < 0 || n_src < 0) return fail(HMX_ERR_ARG, "negative size");
+    if (n_obs > INT32_MAX) return fail(HMX_ERR_ARG, "more than 2^31-1 observation points");
+    if (n_obs == 0) return HMX_OK;
+    if (!easting || !northing || !upward || !out) return fail(HMX_ERR_ARG, "NULL array");
+    if (n_src > 0 && (!p_easting || !p_northing || !p_upward || !coefs))
+        return fail(HMX_ERR_ARG, "NULL source array");
+    int cu = device_cu_count();
+    if (cu < 0) return cu;
+    const int64_t chunks = forward_chunks(n_obs, n_src, cu);
+    const size_t need = (size_t)chunks * (size_t)n_obs * sizeof(double);
+    if (n_src > 0 && (!workspace || workspace_bytes < need))
+        return fail(HMX_ERR_ARG, "workspace too small: %zu < %zu bytes", workspace_bytes, need);
+    hipStream_t s = (hipStream_t)stream;
+    double *partial = static_cast<double *>(workspace);
+    if (dtype == HMX_F64)
+        return predict_typed<double>(easting, northing, upward, n_obs, p_easting, p_northing,
+                                     p_upward, coefs, n_src, out, partial, chunks, s);
+    if (dtype == HMX_F32)
+        return predict_typed<float>(easting, northing, upward, n_obs, p_easting, p_northing,
+                                    p_upward, coefs, n_src, out, partial, chunks, s);
+    return fail(HMX_ERR_ARG, "invalid dtype %d", dtype);
+}
+
+}  // extern "C"

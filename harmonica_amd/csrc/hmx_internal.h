@@ -1,0 +1,69 @@
+// Internal helpers shared by the libhmx translation units: error reporting,
+// device queries and the launch-shape rules of the pairwise kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "../../include/hmx.h"
+
+namespace hmx {
+
+// Record a printf-style message for hmx_last_error() and return `code`.
+int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+
+// Clear the calling thread's last error.
+void clear_error();
+
+// Number of compute units of the current device (cached per device), or a
+// negative status if no device is usable.
+int device_cu_count();
+
+}  // namespace hmx
+
+#define HMX_HIP(expr)                                                              \
+    do {                                                                           \
+        hipError_t hmx_e_ = (expr);                                                \
+        if (hmx_e_ != hipSuccess)                                                  \
+            return ::hmx::fail(HMX_ERR_HIP, "%s failed: %s", #expr,                \
+                               hipGetErrorString(hmx_e_));                         \
+    } while (0)
+
+// After a kernel launch: report launch-configuration errors.
+#define HMX_LAUNCHED(name)                                                         \
+    do {                                                                           \
+        hipError_t hmx_e_ = hipGetLastError();                                     \
+        if (hmx_e_ != hipSuccess)                                                  \
+            return ::hmx::fail(HMX_ERR_HIP, "launch of %s failed: %s", name,       \
+                               hipGetErrorString(hmx_e_));                         \
+    } while (0)
+
+namespace hmx {
+
+// Observation points per workgroup of the pairwise kernels: 4 wavefronts of
+// 64 lanes, one observation point per lane.
+constexpr int kObsPerBlock = 256;
+// Sources staged in LDS per pass.
+constexpr int kTile = 128;
+
+// Split of the source dimension into chunks so that a launch has enough
+// workgroups to fill all compute units several times over.  Each chunk's
+// partial sums go to the workspace and are reduced in chunk order, so the
+// result does not depend on scheduling.
+inline int64_t forward_chunks(int64_t n_obs, int64_t n_src, int n_cu) {
+    int64_t obs_blocks = (n_obs + kObsPerBlock - 1) / kObsPerBlock;
+    if (obs_blocks < 1) obs_blocks = 1;
+    int64_t target = 40LL * (n_cu > 0 ? n_cu : 256);
+    int64_t chunks = (target + obs_blocks - 1) / obs_blocks;
+    int64_t max_by_src = (n_src + kTile - 1) / kTile;
+    if (chunks > max_by_src) chunks = max_by_src;
+    if (chunks > 1024) chunks = 1024;
+    if (chunks < 1) chunks = 1;
+    return chunks;
+}
+
+inline int popcount10(uint32_t m) { return __builtin_popcount(m & HMX_ALL_FIELDS); }
+
+}  // namespace hmx

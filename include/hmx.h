@@ -1,0 +1,162 @@
+/*
+ * hmx.h -- C-ABI of the MI355X (gfx950) pairwise gravity forward-modelling
+ * library (libhmx.so).
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, and
+ * returns an int status (HMX_OK = 0, negative on error; the message is in
+ * hmx_last_error(), thread-local).  Array arguments are DEVICE pointers
+ * (HBM) unless a comment says otherwise; `stream` is a hipStream_t (NULL =
+ * the default stream) and every call is stream-ordered and asynchronous.
+ * Scratch memory ("workspace") is provided by the caller, sized with the
+ * matching *_workspace_bytes query, so no entry point allocates device
+ * memory except hmx_eqs_fit, which documents its own allocations.
+ *
+ * The reference (fatiando/harmonica) has no FFI: its operator interface is
+ * a Numba-jitted loop called with numpy arrays plus a jitted per-pair
+ * function (a "FIELDS" table entry).  Each entry point below replaces one
+ * such loop; the comment names the reference code it replaces.  The Python
+ * host layer (harmonica_amd) keeps the reference's Python API and performs
+ * its validation before calling here.
+ */
+#ifndef HMX_H
+#define HMX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. */
+#define HMX_OK 0
+#define HMX_ERR_ARG (-1)   /* invalid argument (size, pointer, mask, workspace) */
+#define HMX_ERR_HIP (-2)   /* a HIP runtime call failed */
+#define HMX_ERR_NODEV (-3) /* no usable gfx950 device */
+#define HMX_ERR_NUMERIC (-4) /* factorisation failed (matrix not positive definite) */
+
+/* Field bits, in the order of the reference's FIELDS table
+ * (src/harmonica/_forward/prisms/gravity.py:37-48).  Outputs hold one row
+ * of n_obs values per selected field, in this order, in harmonica's units
+ * and signs: potential in J/kg, g_e/g_n/g_z in mGal, tensor terms in
+ * Eotvos, g_z/g_ez/g_nz sign-flipped to the downward convention
+ * (gravity.py:228-235; point.py:253-260). */
+#define HMX_POTENTIAL (1u << 0)
+#define HMX_G_E (1u << 1)
+#define HMX_G_N (1u << 2)
+#define HMX_G_Z (1u << 3)
+#define HMX_G_EE (1u << 4)
+#define HMX_G_NN (1u << 5)
+#define HMX_G_ZZ (1u << 6)
+#define HMX_G_EN (1u << 7)
+#define HMX_G_EZ (1u << 8)
+#define HMX_G_NZ (1u << 9)
+#define HMX_ALL_FIELDS 0x3ffu
+
+/* Coordinate systems of hmx_point_gravity (point.py:231-233). */
+#define HMX_CARTESIAN 0
+#define HMX_SPHERICAL 1
+
+/* Element types of the equivalent-source jacobian/predict
+ * (EquivalentSources(dtype=...), cartesian.py:165-191). */
+#define HMX_F64 0
+#define HMX_F32 1
+
+/* Precision modes of hmx_eqs_fit. */
+#define HMX_FIT_FP64 0      /* J^T W J in FP64, FP64 Cholesky */
+#define HMX_FIT_FP32_MFMA 1 /* J^T W J on FP32 MFMA, FP64 Cholesky + FP64-residual refinement */
+
+/* Last error message of the calling thread ("" if none). */
+const char *hmx_last_error(void);
+
+/* Library version (major * 10000 + minor * 100 + patch). */
+int hmx_version(void);
+
+/* Number of visible gfx950 devices (0 if none; status HMX_OK either way). */
+int hmx_device_count(int *count);
+
+/* Bytes of workspace needed by hmx_prism_gravity, hmx_prism_layer_gravity
+ * and hmx_point_gravity for n_obs observation points, n_sources prisms or
+ * point masses and the given field mask, on the current device. */
+int hmx_forward_workspace_bytes(int64_t n_obs, int64_t n_sources, uint32_t field_mask,
+                                size_t *bytes);
+
+/* Replaces jit_prism_gravity (src/harmonica/_forward/prisms/gravity.py:489-545)
+ * called with the choclo per-pair functions of FIELDS (gravity.py:37-48),
+ * plus the unit/sign post-processing of prism_gravity (gravity.py:227-236).
+ *   easting/northing/upward : (n_obs) observation coordinates [m]
+ *   prisms  : (n_prisms, 6) row-major [west, east, south, north, bottom, top]
+ *   density : (n_prisms) [kg/m^3]
+ *   out     : (popcount(field_mask), n_obs), overwritten
+ *   singular_flags : optional uint32 (may be NULL); bit f is OR-ed in when an
+ *             observation point is a singular point of tensor field f of some
+ *             prism (gravity.py:239-269); that value is NaN.
+ * Null prisms are NOT discarded here (the Python layer does it as
+ * _discard_null_prisms, gravity.py:452-486, does). */
+int hmx_prism_gravity(const double *easting, const double *northing, const double *upward,
+                      int64_t n_obs, const double *prisms, const double *density,
+                      int64_t n_prisms, uint32_t field_mask, double *out,
+                      uint32_t *singular_flags, void *workspace, size_t workspace_bytes,
+                      void *stream);
+
+/* Replaces the singular-point scans _any_singular_point_g_{ee,nn,zz,en,ez,nz}
+ * of _check_singular_points (src/harmonica/_forward/prisms/gravity.py:239-449):
+ * ORs bit f into *flags (device uint32, caller-zeroed) when some observation
+ * point lies on a singular point (vertex or edge, gravity.py:141-152) of
+ * tensor field f of some prism.  Only tensor bits of field_mask are set. */
+int hmx_prism_singular_scan(const double *easting, const double *northing, const double *upward,
+                            int64_t n_obs, const double *prisms, int64_t n_prisms,
+                            uint32_t field_mask, uint32_t *flags, void *stream);
+
+/* Replaces _forward_gravity_prism_layer (src/harmonica/_forward/prisms/layer.py:522-633)
+ * and the post-processing of DatasetAccessorPrismLayer.gravity (layer.py:425-433).
+ *   centre_easting (n_east), centre_northing (n_north): prism centres;
+ *   bottom, top, density: (n_north, n_east) row-major.
+ * Prisms with zero or NaN density, NaN top or bottom, or thickness below
+ * thickness_threshold are skipped, as in the reference.  n_east, n_north >= 2. */
+int hmx_prism_layer_gravity(const double *easting, const double *northing, const double *upward,
+                            int64_t n_obs, const double *centre_easting, int64_t n_east,
+                            const double *centre_northing, int64_t n_north,
+                            const double *bottom, const double *top, const double *density,
+                            double thickness_threshold, uint32_t field_mask, double *out,
+                            uint32_t *singular_flags, void *workspace, size_t workspace_bytes,
+                            void *stream);
+
+/* Replaces point_mass_cartesian / point_mass_spherical
+ * (src/harmonica/_forward/point.py:357-454) with the choclo.point kernels
+ * (point.py:281-316) or the spherical kernels (point.py:324-354), plus the
+ * post-processing of point_gravity (point.py:252-260).
+ * Spherical: coordinates are (longitude [deg], latitude [deg], radius [m]) and
+ * only HMX_POTENTIAL and HMX_G_Z are valid (point.py:306-315). */
+int hmx_point_gravity(const double *c0, const double *c1, const double *c2, int64_t n_obs,
+                      const double *p0, const double *p1, const double *p2,
+                      const double *masses, int64_t n_points, int coordinate_system,
+                      uint32_t field_mask, double *out, void *workspace,
+                      size_t workspace_bytes, void *stream);
+
+/* Replaces jacobian_numba_{serial,parallel} (src/harmonica/_equivalent_sources/utils.py:54-74)
+ * with greens_func_cartesian (cartesian.py:634-644):
+ *   jac[i * ld + j] = 1 / |x_i - p_j|, i < n_data, j < n_src.
+ * dtype HMX_F64: all pointers double; HMX_F32: coordinates and jac float. */
+int hmx_eqs_jacobian(const void *easting, const void *northing, const void *upward,
+                     int64_t n_data, const void *p_easting, const void *p_northing,
+                     const void *p_upward, int64_t n_src, int dtype, void *jac, int64_t ld,
+                     void *stream);
+
+/* Bytes of workspace needed by hmx_eqs_predict on the current device. */
+int hmx_eqs_predict_workspace_bytes(int64_t n_obs, int64_t n_src, size_t *bytes);
+
+/* Replaces predict_numba_{serial,parallel} (src/harmonica/_equivalent_sources/utils.py:77-97):
+ *   out[i] = sum_j coefs[j] / |x_i - p_j|  (out overwritten).
+ * dtype HMX_F64: all double.  HMX_F32: coordinates, coefs, out float; the
+ * pair terms are evaluated in FP32 and accumulated in FP64. */
+int hmx_eqs_predict(const void *easting, const void *northing, const void *upward,
+                    int64_t n_obs, const void *p_easting, const void *p_northing,
+                    const void *p_upward, const void *coefs, int64_t n_src, int dtype,
+                    void *out, void *workspace, size_t workspace_bytes, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HMX_H */

@@ -1,0 +1,251 @@
+"""
+GPU parity of the forward models against the CPU oracle, through the
+drop-in API and the C-ABI.  Tolerance: 1e-10 x max|field| for FP64 results
+(the reference's FP64 path and this one sum the same closed forms in a
+different order with different libm roundings); NaN at the same places.
+"""
+
+import numpy as np
+import numpy.testing as npt
+import pytest
+
+import oracle
+from conftest import grid_coordinates
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = oracle.FIELDS
+TENSORS = ("g_ee", "g_nn", "g_zz", "g_en", "g_ez", "g_nz")
+RTOL_MAX = 1e-10
+
+
+@pytest.fixture(scope="module")
+def hm():
+    import torch
+
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import harmonica_amd
+
+    return harmonica_amd
+
+
+def assert_parity(got, expected, tol=RTOL_MAX, err_msg=""):
+    got, expected = np.asarray(got), np.asarray(expected)
+    assert got.shape == expected.shape, err_msg
+    npt.assert_array_equal(np.isnan(got), np.isnan(expected), err_msg=err_msg)
+    finite = np.isfinite(expected)
+    scale = np.abs(expected[finite]).max() if finite.any() else 1.0
+    npt.assert_allclose(got[finite], expected[finite], rtol=0, atol=tol * scale, err_msg=err_msg)
+
+
+def test_doctest_values(hm):
+    """gravity.py:182-193."""
+    coords = ([-40, 0, 40], [0, 0, 0], [30, 30, 30])
+    gz = hm.prism_gravity(coords, [-34, 5, -18, 14, -345, -146], 2670, field="g_z")
+    assert "({:.5f}, {:.5f}, {:.5f})".format(*gz) == "(0.06552, 0.06629, 0.06174)"
+    prisms = [[-134, -5, -45, 45, -200, -50], [5, 134, -45, 45, -180, -30]]
+    gz = hm.prism_gravity(coords, prisms, [-300, 300], field="g_z")
+    assert "({:.5f}, {:.5f}, {:.5f})".format(*gz) == "(-0.05380, 0.02908, 0.11237)"
+
+
+def _random_model(seed, n_prisms=57, n_obs=300):
+    rng = np.random.default_rng(seed)
+    w = rng.uniform(-2000, 1500, n_prisms)
+    s = rng.uniform(-2000, 1500, n_prisms)
+    b = rng.uniform(-1500, -100, n_prisms)
+    prisms = np.column_stack([w, w + rng.uniform(10, 600, n_prisms), s,
+                              s + rng.uniform(10, 600, n_prisms), b, b + rng.uniform(10, 900, n_prisms)])
+    dens = rng.normal(0, 400, n_prisms)
+    # points around, above, inside and below the prisms
+    coords = (rng.uniform(-2500, 2500, n_obs), rng.uniform(-2500, 2500, n_obs),
+              rng.uniform(-2000, 600, n_obs))
+    return coords, prisms, dens
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+@pytest.mark.parametrize("field", FIELDS)
+def test_prism_gravity_vs_oracle(hm, field, seed):
+    coords, prisms, dens = _random_model(seed)
+    got = hm.prism_gravity(coords, prisms, dens, field=field)
+    assert_parity(got, oracle.prism_gravity(coords, prisms, dens, field), err_msg=field)
+
+
+def test_singular_points_nan_and_warning(hm):
+    """test_prism.py:380-502 geometry: vertices and edge centres."""
+    p = np.array([-10.3, 5.4, 8.6, 14.3, -30.3, 2.4])
+    e, n, u = (c.ravel() for c in np.meshgrid(p[:2], p[2:4], p[4:6]))
+    # add ordinary points so that only the singular ones are NaN
+    coords = (np.r_[e, 100.0, -3.0], np.r_[n, 7.0, 11.0], np.r_[u, 5.0, 40.0])
+    for field in FIELDS:
+        if field in TENSORS:
+            with pytest.warns(UserWarning, match="Found observation point on singular point"):
+                got = hm.prism_gravity(coords, p, [2900.0], field=field)
+        else:
+            got = hm.prism_gravity(coords, p, [2900.0], field=field)
+        assert_parity(got, oracle.prism_gravity(coords, p, [2900.0], field), err_msg=field)
+    # edge centres: NaN only on the component's singular families
+    for direction, pts in {
+        "e": (np.full(4, (p[0] + p[1]) / 2), *(c.ravel() for c in np.meshgrid(p[2:4], p[4:6]))),
+    }.items():
+        for field in ("g_nn", "g_zz", "g_nz"):
+            with pytest.warns(UserWarning):
+                got = hm.prism_gravity(pts, p, [2900.0], field=field)
+            assert np.isnan(got).all()
+
+
+def test_no_warning_when_checks_disabled(hm):
+    import warnings
+
+    p = [0, 10, 0, 10, -10, 0]
+    with warnings.catch_warnings():
+        warnings.simplefilter("error")
+        res = hm.prism_gravity((0, 0, 0), p, 100, field="g_zz", disable_checks=True)
+    assert np.isnan(res)
+
+
+def test_null_prisms_and_inverted(hm):
+    """test_prism.py:94-158."""
+    coords = grid_coordinates((-50, 50, -50, 50), shape=(3, 3), extra=0)
+    prisms = [
+        [-100, 100, -200, 200, -10e3, -5e3],
+        [100, 200, 200, 300, -10e3, -4e3],
+        [30, 30, -200, 200, -10e3, -5e3],
+        [100, 200, 30, 30, -10e3, -5e3],
+        [-100, 100, -200, 200, -10e3, -10e3],
+    ]
+    density = [2600, 0, 3000, 3100, 3200]
+    npt.assert_allclose(hm.prism_gravity(coords, prisms, density, field="g_z"),
+                        hm.prism_gravity(coords, [prisms[0]], [density[0]], field="g_z"))
+    inv = hm.prism_gravity([0, 0, 0], [0, 10, 0, 10, 0, -10], 100, field="potential",
+                           disable_checks=True)
+    val = hm.prism_gravity([0, 0, 0], [0, 10, 0, 10, -10, 0], 100, field="potential")
+    npt.assert_allclose(inv, -val)
+
+
+def test_float32_dtype(hm):
+    coords, prisms, dens = _random_model(3, 20, 50)
+    got = hm.prism_gravity(coords, prisms, dens, field="g_z", dtype="float32")
+    assert got.dtype == np.float32
+    npt.assert_allclose(got, oracle.prism_gravity(coords, prisms, dens, "g_z"), rtol=1e-5,
+                        atol=1e-6 * np.abs(got).max())
+
+
+def test_fused_all_fields_match_single(hm):
+    """The fused ten-field launch equals ten single-field launches."""
+    from harmonica_amd._forward import prism_fields_device, to_device
+
+    coords, prisms, dens = _random_model(4, 200, 1000)
+    dev = [to_device(c) for c in coords]
+    P, D = to_device(prisms), to_device(dens)
+    fused = prism_fields_device(*dev, P, D, FIELDS).cpu().numpy()
+    for k, f in enumerate(FIELDS):
+        single = prism_fields_device(*dev, P, D, (f,)).cpu().numpy()[0]
+        assert_parity(fused[k], single, tol=1e-13, err_msg=f)
+        assert_parity(fused[k], oracle.prism_gravity(coords, prisms, dens, f), err_msg=f)
+    # an arbitrary subset goes through per-field launches
+    sub = ("g_z", "potential", "g_en")
+    res = prism_fields_device(*dev, P, D, sub).cpu().numpy()
+    for k, f in enumerate(("potential", "g_z", "g_en")):
+        assert_parity(res[k], oracle.prism_gravity(coords, prisms, dens, f), err_msg=f)
+
+
+def _layer_inputs(n_e=23, n_n=17, seed=5):
+    rng = np.random.default_rng(seed)
+    easting = np.linspace(-1000, 1200, n_e)
+    northing = np.linspace(300, 2000, n_n)
+    surface = rng.uniform(-300, 800, (n_n, n_e))
+    density = rng.uniform(2000, 3000, (n_n, n_e))
+    return easting, northing, surface, density
+
+
+@pytest.mark.parametrize("field", FIELDS)
+def test_layer_vs_oracle(hm, field):
+    easting, northing, surface, density = _layer_inputs()
+    density[3, 4] = np.nan
+    surface[5, 6] = np.nan
+    density[7, 8] = 0.0
+    layer = hm.prism_layer((easting, northing), surface, 0.0, properties={"density": density})
+    coords = grid_coordinates((-1200, 1400, 200, 2100), shape=(21, 19), extra=900.0)
+    with pytest.warns(UserWarning, match="Found NaN values in 'density'"):
+        got = layer.prism_layer.gravity(coords, field=field)
+    expected = oracle.prism_layer_gravity(coords, easting, northing, layer.bottom.values,
+                                          layer.top.values, density, field)
+    assert_parity(got, expected, err_msg=field)
+    # and the reference's own consistency test (test_prism_layer.py:240-261)
+    dens_clean = np.nan_to_num(density, nan=0.0)
+    layer2 = hm.prism_layer((easting, northing), np.nan_to_num(surface), 0.0,
+                            properties={"density": dens_clean})
+    via_prisms = hm.prism_gravity(coords, layer2.prism_layer._to_prisms(), dens_clean.ravel(), field=field)
+    assert_parity(layer2.prism_layer.gravity(coords, field=field), via_prisms, tol=1e-12,
+                  err_msg=field)
+
+
+def test_layer_thickness_threshold(hm):
+    easting, northing, surface, density = _layer_inputs()
+    layer = hm.prism_layer((easting, northing), surface, 0.0, properties={"density": density})
+    coords = grid_coordinates((-1200, 1400, 200, 2100), shape=(9, 8), extra=900.0)
+    prisms = layer.prism_layer._to_prisms()
+    keep = np.abs(prisms[:, 5] - prisms[:, 4]) >= 120.0
+    npt.assert_allclose(layer.prism_layer.gravity(coords, "g_z", thickness_threshold=120.0),
+                        hm.prism_gravity(coords, prisms[keep], density.ravel()[keep], "g_z"),
+                        rtol=1e-12)
+
+
+def test_config1_layer_parity(hm):
+    """100 x 100 prism layer (500 m prisms on a 50 km square, sinusoidal tops)
+    against a 64 x 64 grid at 1000 m: g_z FP64, 4.096e7 pairs."""
+    from harmonica_amd.workloads import terrain_layer_small
+
+    w = terrain_layer_small()
+    layer = hm.prism_layer((w["easting"], w["northing"]), w["surface"], 0.0,
+                           properties={"density": w["density"]})
+    got = layer.prism_layer.gravity(w["coordinates"], field="g_z")
+    expected = oracle.prism_layer_gravity(w["coordinates"], w["easting"], w["northing"],
+                                          layer.bottom.values, layer.top.values, w["density"], "g_z")
+    assert_parity(got, expected)
+
+
+# ----------------------------------------------------------- point masses
+@pytest.mark.parametrize("field", FIELDS + ("g_ne", "g_ze", "g_zn"))
+def test_point_cartesian_vs_oracle(hm, field):
+    rng = np.random.default_rng(11)
+    pts = (rng.uniform(-1e3, 1e3, 333), rng.uniform(-1e3, 1e3, 333), rng.uniform(-900, -100, 333))
+    masses = rng.normal(0, 1e8, 333)
+    coords = grid_coordinates((-1500, 1500, -1500, 1500), shape=(25, 31), extra=50.0)
+    assert_parity(hm.point_gravity(coords, pts, masses, field),
+                  oracle.point_gravity(coords, pts, masses, field), err_msg=field)
+
+
+def test_point_csv_fixture(hm):
+    import os
+
+    here = os.path.dirname(os.path.abspath(__file__))
+    e, n, u, pot = np.loadtxt(os.path.join(here, "golden", "sample_point_gravity.csv"),
+                              delimiter=",", unpack=True)
+    npt.assert_allclose(hm.point_gravity((e, n, u), [0, 0, 0], [5000], "potential"), pot)
+
+
+@pytest.mark.parametrize("field", ["potential", "g_z"])
+def test_point_spherical_vs_oracle(hm, field):
+    rng = np.random.default_rng(12)
+    pts = (rng.uniform(2, 10, 80), rng.uniform(-3, 5, 80), np.full(80, 6400e3 - 10e3))
+    masses = np.arange(80.0)
+    lon, lat = np.meshgrid(np.linspace(2, 10, 17), np.linspace(-3, 5, 13))
+    coords = (lon, lat, np.full_like(lon, 6400e3))
+    assert_parity(hm.point_gravity(coords, pts, masses, field, coordinate_system="spherical"),
+                  oracle.point_gravity(coords, pts, masses, field, "spherical"), err_msg=field)
+
+
+def test_point_fused_all_fields(hm):
+    from harmonica_amd._forward import point_fields_device, to_device
+
+    rng = np.random.default_rng(13)
+    pts = [to_device(rng.uniform(-1e3, 1e3, 200)) for _ in range(2)] + [to_device(rng.uniform(-900, -100, 200))]
+    m = rng.normal(0, 1e8, 200)
+    coords = grid_coordinates((-1500, 1500, -1500, 1500), shape=(16, 16), extra=10.0)
+    dc = [to_device(np.asarray(c).ravel()) for c in coords]
+    fused = point_fields_device(*dc, *pts, to_device(m), FIELDS).cpu().numpy()
+    hp = [p.cpu().numpy() for p in pts]
+    for k, f in enumerate(FIELDS):
+        assert_parity(fused[k], oracle.point_gravity(coords, hp, m, f).ravel(), err_msg=f)
