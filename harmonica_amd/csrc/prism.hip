@@ -93,6 +93,8 @@ __global__ __launch_bounds__(kObsPerBlock) void prism_forward_kernel(
     uint32_t *__restrict__ flags) {
     using F = FieldSet<M>;
     __shared__ StagedPrism tile[kTile];
+    __shared__ dm::Tables tables;
+    dm::load_tables(&tables);
     const int i = blockIdx.x * kObsPerBlock + threadIdx.x;
     const bool valid = i < n_obs;
     const int ii = valid ? i : n_obs - 1;
@@ -111,7 +113,8 @@ __global__ __launch_bounds__(kObsPerBlock) void prism_forward_kernel(
         for (int p = 0; p < cnt; ++p) {
             const StagedPrism sp = tile[p];
             if (Src::may_skip && sp.skip != 0.0) continue;
-            dev::prism_pair<M>(e, n, u, sp.w, sp.e, sp.s, sp.n, sp.b, sp.t, sp.grho, acc, sflags);
+            dev::prism_pair<M>(e, n, u, sp.w, sp.e, sp.s, sp.n, sp.b, sp.t, sp.grho, acc, sflags,
+                               tables);
         }
     }
     if (valid) {

@@ -23,11 +23,12 @@
 
 #include <cstdint>
 
+#include "devmath.h"
+
 namespace hmx {
 namespace dev {
 
 constexpr double kG = 6.6743e-11;  // harmonica/constants.py:12
-constexpr double kHalfPi = 1.57079632679489661923;
 
 // Compile-time description of a set of fields (bit f = FIELDS[f]).
 template <unsigned M>
@@ -48,44 +49,40 @@ struct FieldSet {
 };
 
 // Principal value of atan(a / b); b == 0 gives +-pi/2 by the sign of a, or 0.
-__device__ __forceinline__ double safe_atan(double a, double b) {
-    double t = atan(a / b);
-    double special = a > 0.0 ? kHalfPi : (a < 0.0 ? -kHalfPi : 0.0);
-    return b != 0.0 ? t : special;
+__device__ __forceinline__ double safe_atan(double a, double b, const dm::Tables &tab) {
+    return dm::atan_ratio(a, b, tab);
 }
 
 // ln(x + r); the x < 0 branch uses (y^2 + z^2) / (r - x); 0 where the
 // argument is 0.
-__device__ __forceinline__ double safe_log(double x, double y, double z, double r) {
+__device__ __forceinline__ double safe_log(double x, double y, double z, double r,
+                                           const dm::Tables &tab) {
     double arg;
     if (x < 0.0) {
-        arg = (y * y + z * z) / (r - x);
+        arg = dm::div(fma(y, y, z * z), r - x);
     } else {
         arg = x + r;
     }
-    double l = log(arg);
+    const double l = dm::log_pos(arg, tab);
     return arg == 0.0 ? 0.0 : l;
 }
 
-// Accumulate into vs[] the signed kernel values of one vertex.
-template <unsigned M, bool NEG>
+// Accumulate into vs[] the kernel values of one vertex times its sign sg
+// (+1 or -1).
+template <unsigned M>
 __device__ __forceinline__ void vertex(double x, double y, double z, double x2, double y2,
-                                       double z2, double (&vs)[FieldSet<M>::count]) {
+                                       double z2, double sg, double (&vs)[FieldSet<M>::count],
+                                       const dm::Tables &tab) {
     using F = FieldSet<M>;
-    const double r = sqrt(x2 + y2 + z2);
+    const double r = dm::sqrt_nn(x2 + y2 + z2);
     double Lx = 0.0, Ly = 0.0, Lz = 0.0, Ax = 0.0, Ay = 0.0, Az = 0.0;
-    if constexpr (F::Lx) Lx = safe_log(x, y, z, r);
-    if constexpr (F::Ly) Ly = safe_log(y, z, x, r);
-    if constexpr (F::Lz) Lz = safe_log(z, x, y, r);
-    if constexpr (F::Ax) Ax = safe_atan(y * z, x * r);
-    if constexpr (F::Ay) Ay = safe_atan(z * x, y * r);
-    if constexpr (F::Az) Az = safe_atan(x * y, z * r);
-    auto add = [&](int f, double k) {
-        if (NEG)
-            vs[F::slot(f)] -= k;
-        else
-            vs[F::slot(f)] += k;
-    };
+    if constexpr (F::Lx) Lx = safe_log(x, y, z, r, tab);
+    if constexpr (F::Ly) Ly = safe_log(y, z, x, r, tab);
+    if constexpr (F::Lz) Lz = safe_log(z, x, y, r, tab);
+    if constexpr (F::Ax) Ax = safe_atan(y * z, x * r, tab);
+    if constexpr (F::Ay) Ay = safe_atan(z * x, y * r, tab);
+    if constexpr (F::Az) Az = safe_atan(x * y, z * r, tab);
+    auto add = [&](int f, double k) { vs[F::slot(f)] = fma(sg, k, vs[F::slot(f)]); };
     if constexpr (F::pot) {
         double k = 0.0;
         k += (x != 0.0 && y != 0.0) ? x * y * Lz : 0.0;
@@ -107,12 +104,17 @@ __device__ __forceinline__ void vertex(double x, double y, double z, double x2, 
     if constexpr (F::nu) add(9, Lx);
 }
 
+#ifndef HMX_VERTEX_UNROLL
+#define HMX_VERTEX_UNROLL 2
+#endif
+
 // Add the fields of one prism (bounds p[0..5] = W, E, S, N, B, T; grho =
 // G * density) at (e, n, u) to acc[]; OR singular-point bits into *sflags.
 template <unsigned M>
 __device__ __forceinline__ void prism_pair(double e, double n, double u, double W, double E,
                                            double S, double N, double B, double T, double grho,
-                                           double (&acc)[FieldSet<M>::count], uint32_t &sflags) {
+                                           double (&acc)[FieldSet<M>::count], uint32_t &sflags,
+                                           const dm::Tables &tab) {
     using F = FieldSet<M>;
     double vs[F::count];
 #pragma unroll
@@ -123,15 +125,18 @@ __device__ __forceinline__ void prism_pair(double e, double n, double u, double 
     const double x0s = x0 * x0, x1s = x1 * x1, y0s = y0 * y0, y1s = y1 * y1;
     const double z0s = z0 * z0, z1s = z1 * z1;
     // Vertex order of the reference's loop: east, north, top first; sign
-    // (-1)^(i + j + k).
-    vertex<M, false>(x0, y0, z0, x0s, y0s, z0s, vs);
-    vertex<M, true>(x0, y0, z1, x0s, y0s, z1s, vs);
-    vertex<M, true>(x0, y1, z0, x0s, y1s, z0s, vs);
-    vertex<M, false>(x0, y1, z1, x0s, y1s, z1s, vs);
-    vertex<M, true>(x1, y0, z0, x1s, y0s, z0s, vs);
-    vertex<M, false>(x1, y0, z1, x1s, y0s, z1s, vs);
-    vertex<M, false>(x1, y1, z0, x1s, y1s, z0s, vs);
-    vertex<M, true>(x1, y1, z1, x1s, y1s, z1s, vs);
+    // (-1)^(i + j + k).  A partially unrolled loop keeps two vertices in
+    // flight, which bounds register pressure (occupancy) while leaving the
+    // scheduler independent work.
+#pragma unroll HMX_VERTEX_UNROLL
+    for (int v = 0; v < 8; ++v) {
+        const bool bi = v & 4, bj = v & 2, bk = v & 1;
+        const double x = bi ? x1 : x0, xs = bi ? x1s : x0s;
+        const double y = bj ? y1 : y0, ys = bj ? y1s : y0s;
+        const double z = bk ? z1 : z0, zs = bk ? z1s : z0s;
+        const double sg = (__builtin_popcount(v) & 1) ? -1.0 : 1.0;
+        vertex<M>(x, y, z, xs, ys, zs, sg, vs, tab);
+    }
     if constexpr (F::tensor) {
         const bool in_e = W <= e && e <= E, in_n = S <= n && n <= N, in_u = B <= u && u <= T;
         const bool at_e = e == W || e == E, at_n = n == S || n == N, at_u = u == B || u == T;

@@ -155,6 +155,17 @@ int hmx_eqs_predict(const void *easting, const void *northing, const void *upwar
                     const void *p_upward, const void *coefs, int64_t n_src, int dtype,
                     void *out, void *workspace, size_t workspace_bytes, void *stream);
 
+/* Device math routines exposed for accuracy testing (no counterpart in the
+ * reference; these are the FP64 sqrt, division, logarithm and
+ * principal-value arctangent used inside the pairwise kernels):
+ *   out[i] = f(a[i], b[i]),  b unused for HMX_DM_SQRT and HMX_DM_LOG. */
+#define HMX_DM_SQRT 0       /* sqrt(a), a >= 0 */
+#define HMX_DM_DIV 1        /* a / b */
+#define HMX_DM_LOG 2        /* ln(a), a > 0 */
+#define HMX_DM_ATAN_RATIO 3 /* atan(a / b), +-pi/2 when b == 0 */
+int hmx_devmath_eval(int fn, const double *a, const double *b, double *out, int64_t n,
+                     void *stream);
+
 #ifdef __cplusplus
 }
 #endif
