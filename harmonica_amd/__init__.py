@@ -15,6 +15,7 @@ streams.  There is no CPU fallback: without the library or a GPU the
 functions raise.
 """
 
+from ._eqs import EquivalentSources
 from ._forward import point_gravity, prism_gravity
 from ._layer import DatasetAccessorPrismLayer, prism_layer
 from .constants import GRAVITATIONAL_CONST
@@ -22,6 +23,7 @@ from .constants import GRAVITATIONAL_CONST
 __all__ = [
     "GRAVITATIONAL_CONST",
     "DatasetAccessorPrismLayer",
+    "EquivalentSources",
     "point_gravity",
     "prism_gravity",
     "prism_layer",

@@ -104,9 +104,10 @@ __device__ __forceinline__ void vertex(double x, double y, double z, double x2, 
     if constexpr (F::nu) add(9, Lx);
 }
 
-#ifndef HMX_VERTEX_UNROLL
-#define HMX_VERTEX_UNROLL 2
-#endif
+// Vertices evaluated per loop iteration: 4 for one field (register use
+// allows 3 waves per SIMD), 2 for several fields (more live accumulators).
+template <unsigned M>
+constexpr int kVertexUnroll = FieldSet<M>::count == 1 ? 4 : 2;
 
 // Add the fields of one prism (bounds p[0..5] = W, E, S, N, B, T; grho =
 // G * density) at (e, n, u) to acc[]; OR singular-point bits into *sflags.
@@ -125,10 +126,9 @@ __device__ __forceinline__ void prism_pair(double e, double n, double u, double 
     const double x0s = x0 * x0, x1s = x1 * x1, y0s = y0 * y0, y1s = y1 * y1;
     const double z0s = z0 * z0, z1s = z1 * z1;
     // Vertex order of the reference's loop: east, north, top first; sign
-    // (-1)^(i + j + k).  A partially unrolled loop keeps two vertices in
-    // flight, which bounds register pressure (occupancy) while leaving the
-    // scheduler independent work.
-#pragma unroll HMX_VERTEX_UNROLL
+    // (-1)^(i + j + k).  A partially unrolled loop bounds register pressure
+    // (occupancy) while leaving the scheduler independent vertices.
+#pragma unroll kVertexUnroll<M>
     for (int v = 0; v < 8; ++v) {
         const bool bi = v & 4, bj = v & 2, bk = v & 1;
         const double x = bi ? x1 : x0, xs = bi ? x1s : x0s;

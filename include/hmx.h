@@ -9,7 +9,7 @@
  * the default stream) and every call is stream-ordered and asynchronous.
  * Scratch memory ("workspace") is provided by the caller, sized with the
  * matching *_workspace_bytes query, so no entry point allocates device
- * memory except hmx_eqs_fit, which documents its own allocations.
+ * memory.
  *
  * The reference (fatiando/harmonica) has no FFI: its operator interface is
  * a Numba-jitted loop called with numpy arrays plus a jitted per-pair
@@ -63,8 +63,8 @@ extern "C" {
 #define HMX_F32 1
 
 /* Precision modes of hmx_eqs_fit. */
-#define HMX_FIT_FP64 0      /* J^T W J in FP64, FP64 Cholesky */
-#define HMX_FIT_FP32_MFMA 1 /* J^T W J on FP32 MFMA, FP64 Cholesky + FP64-residual refinement */
+#define HMX_FIT_FP64 0      /* J^T W J on FP64 MFMA, FP64 Cholesky, direct solve */
+#define HMX_FIT_FP32_MFMA 1 /* J^T W J on FP32 MFMA, FP64 Cholesky, FP64-residual CG */
 
 /* Last error message of the calling thread ("" if none). */
 const char *hmx_last_error(void);
@@ -154,6 +154,37 @@ int hmx_eqs_predict(const void *easting, const void *northing, const void *upwar
                     int64_t n_obs, const void *p_easting, const void *p_northing,
                     const void *p_upward, const void *coefs, int64_t n_src, int dtype,
                     void *out, void *workspace, size_t workspace_bytes, void *stream);
+
+/* Bytes of workspace needed by hmx_eqs_fit (the n_src x n_src normal matrix
+ * and its factor, the n_data x n_src scaled Jacobian in the mode's
+ * precision, inverses of the diagonal Cholesky blocks and vectors). */
+int hmx_eqs_fit_workspace_bytes(int64_t n_data, int64_t n_src, int precision_mode,
+                                size_t *bytes);
+
+/* Replaces the fit path of EquivalentSources.fit
+ * (src/harmonica/_equivalent_sources/cartesian.py:279-280): the jacobian
+ * (utils.py:54-74) followed by verde.base.least_squares (third-party, verde
+ * >= 1.8.1): column scaling by the population standard deviation of J
+ * (sklearn StandardScaler(with_mean=False)), weighted ridge normal
+ * equations (sklearn Ridge(alpha=damping, fit_intercept=False); damping 0
+ * stands for damping=None), unscaling of the coefficients.
+ *   data, weights (may be NULL): (n_data) device arrays;
+ *   precision_mode: HMX_FIT_FP64 (FP64 MFMA normal matrix, FP64 Cholesky,
+ *     direct solve) or HMX_FIT_FP32_MFMA (FP32 Jacobian and FP32-MFMA normal
+ *     matrix, shifted, factorised in FP64 and used as the preconditioner of
+ *     conjugate gradients on the exact FP64 normal equations);
+ *   max_iter, rtol: CG limit and stop rule (relative change of the
+ *     data-space prediction per iteration);
+ *   coefs: (n_src) device output;
+ *   info: HOST array of 5 doubles: [CG iterations, last relative update,
+ *     shift tau, factorisation retries, max diagonal of the normal matrix].
+ * Synchronous: returns after the coefficients are written. */
+int hmx_eqs_fit(const double *easting, const double *northing, const double *upward,
+                int64_t n_data, const double *data, const double *weights,
+                const double *p_easting, const double *p_northing, const double *p_upward,
+                int64_t n_src, double damping, int precision_mode, int max_iter, double rtol,
+                double *coefs, double *info, void *workspace, size_t workspace_bytes,
+                void *stream);
 
 /* Device math routines exposed for accuracy testing (no counterpart in the
  * reference; these are the FP64 sqrt, division, logarithm and

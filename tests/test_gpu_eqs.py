@@ -1,0 +1,231 @@
+"""
+GPU parity of the equivalent-source path: jacobian and predict against the
+CPU oracle, the GPU least-squares fit against the verde/sklearn restatement
+(oracle.eqs_fit_coefs, itself pinned against scikit-learn), the
+reference's accuracy tests (test_eq_sources_cartesian.py:111-212,255-404),
+and the mixed-precision fit against the FP64 fit.
+"""
+
+import numpy as np
+import numpy.testing as npt
+import pytest
+
+import oracle
+from conftest import grid_coordinates
+
+pytestmark = pytest.mark.gpu
+REGION = (-3e3, -1e3, 5e3, 7e3)
+
+
+@pytest.fixture(scope="module")
+def hm():
+    import torch
+
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import harmonica_amd
+
+    return harmonica_amd
+
+
+def checkerboard(amplitude, region, easting, northing):
+    """Synthetic checkerboard (verde.synthetic.CheckerBoard): wavelengths of
+    half the region size in each direction."""
+    w_e = (region[1] - region[0]) / 2
+    w_n = (region[3] - region[2]) / 2
+    return amplitude * np.sin(2 * np.pi * easting / w_e) * np.cos(2 * np.pi * northing / w_n)
+
+
+@pytest.fixture(scope="module")
+def synthetic():
+    points = grid_coordinates(REGION, shape=(6, 6), extra=-1e3)
+    masses = checkerboard(1e13, REGION, points[0], points[1])
+    coords = grid_coordinates(REGION, shape=(40, 40), extra=0)
+    data = oracle.point_gravity(coords, points, masses, "g_z")
+    return points, masses, coords, data
+
+
+def _scattered(seed, n_data, n_src, depth=1500.0):
+    rng = np.random.default_rng(seed)
+    coords = (rng.uniform(0, 5e3, n_data), rng.uniform(0, 5e3, n_data), rng.uniform(0, 100, n_data))
+    pts = (rng.uniform(0, 5e3, n_src), rng.uniform(0, 5e3, n_src), np.full(n_src, -depth))
+    return coords, pts, rng
+
+
+@pytest.mark.parametrize("dtype", ["float64", "float32"])
+def test_jacobian_vs_oracle(hm, dtype):
+    coords, pts, _ = _scattered(0, 333, 211)
+    eqs = hm.EquivalentSources(dtype=dtype)
+    jac = eqs.jacobian(coords, pts)
+    assert jac.dtype == np.dtype(dtype)
+    rtol = 1e-15 if dtype == "float64" else 1e-6
+    npt.assert_allclose(jac, oracle.eqs_jacobian(coords, pts), rtol=rtol)
+
+
+def test_jacobian_symmetry(hm):
+    """test_eq_sources_cartesian.py:293-318."""
+    e, n = grid_coordinates((-100, 100, -100, 100), shape=(2, 2))
+    coords = (e.ravel(), n.ravel(), np.zeros(4))
+    pts = (e.ravel(), n.ravel(), np.full(4, 100.0))
+    jac = hm.EquivalentSources().jacobian(coords, pts)
+    diag = np.diag_indices(4)
+    npt.assert_allclose(jac[diag], jac[diag][0])
+    anti = (diag[0], diag[1][::-1])
+    npt.assert_allclose(jac[anti], jac[anti][0])
+    nn = np.ones((4, 4), bool)
+    nn[diag] = False
+    nn[anti] = False
+    npt.assert_allclose(jac[nn], jac[nn][0])
+
+
+@pytest.mark.parametrize("dtype", ["float64", "float32"])
+def test_predict_vs_oracle(hm, dtype):
+    from harmonica_amd._eqs import predict_device
+    import torch
+
+    coords, pts, rng = _scattered(1, 1000, 700)
+    coefs = rng.normal(size=700)
+    tdtype = torch.float64 if dtype == "float64" else torch.float32
+    got = predict_device(tuple(c.astype(dtype) for c in coords), tuple(p.astype(dtype) for p in pts),
+                         coefs.astype(dtype), dtype=tdtype).cpu().numpy()
+    expected = oracle.eqs_predict(coords, pts, coefs)
+    tol = 1e-12 if dtype == "float64" else 1e-5
+    npt.assert_allclose(got, expected, atol=tol * np.abs(expected).max())
+
+
+@pytest.mark.parametrize("damping", [None, 1e-3, 0.1])
+@pytest.mark.parametrize("weighted", [False, True])
+def test_fit_fp64_vs_least_squares_restatement(hm, damping, weighted):
+    """GPU fit (scaled columns, weighted ridge normal equations, FP64 MFMA
+    normal matrix and Cholesky) against the restatement of verde's
+    least_squares checked against scikit-learn."""
+    from harmonica_amd._eqs import fit_device
+
+    coords, pts, rng = _scattered(2, 700, 300)
+    J = oracle.eqs_jacobian(coords, pts)
+    data = J @ rng.normal(size=300) + 1e-3 * rng.normal(size=700)
+    w = rng.uniform(0.5, 2.0, 700) if weighted else None
+    coefs, info = fit_device(coords, data, w, pts, damping)
+    expected = oracle.eqs_fit_coefs(J, data, w, damping)
+    pred, ref = J @ coefs, J @ expected
+    npt.assert_allclose(pred, ref, atol=1e-9 * np.abs(ref).max())
+    if damping is not None:
+        npt.assert_allclose(coefs, expected, rtol=1e-7, atol=1e-9 * np.abs(expected).max())
+
+
+def test_fit_multi_block_sizes(hm):
+    """Sizes that are not multiples of the 128-wide Cholesky and MFMA tiles,
+    spanning several blocks (exercises the blocked factorisation and the
+    triangular solves end to end)."""
+    from harmonica_amd._eqs import fit_device
+
+    for n_data, n_src in [(520, 389), (300, 300), (1000, 129)]:
+        coords, pts, rng = _scattered(3, n_data, n_src, depth=800.0)
+        J = oracle.eqs_jacobian(coords, pts)
+        data = J @ rng.normal(size=n_src)
+        coefs, _ = fit_device(coords, data, None, pts, 1e-2)
+        expected = oracle.eqs_fit_coefs(J, data, None, 1e-2)
+        npt.assert_allclose(coefs, expected, rtol=1e-7, atol=1e-9 * np.abs(expected).max(),
+                            err_msg=f"{n_data} x {n_src}")
+
+
+@pytest.mark.parametrize("dtype", ["default", "float32"])
+def test_equivalent_sources_cartesian(hm, synthetic, dtype):
+    """test_eq_sources_cartesian.py:111-152."""
+    points, masses, coords, data = synthetic
+    atol = 1e-3 * np.abs(data).max()
+    kwargs = {"dtype": dtype} if dtype != "default" else {}
+    eqs = hm.EquivalentSources(damping=1e-12, **kwargs).fit(coords, data)
+    npt.assert_allclose(data, eqs.predict(coords), atol=atol)
+    grid_coords = grid_coordinates(REGION, shape=(60, 60), extra=0)
+    true = oracle.point_gravity(grid_coords, points, masses, "g_z")
+    npt.assert_allclose(true, eqs.predict(grid_coords), atol=atol)
+    npt.assert_allclose(true, eqs.grid(grid_coords).scalars, atol=atol)
+    profile = eqs.profile((REGION[0], REGION[2]), (REGION[0], REGION[3]), 0, 60)
+    true = oracle.point_gravity((profile.easting, profile.northing, profile.upward), points,
+                                masses, "g_z")
+    npt.assert_allclose(true, profile.scalars, atol=atol)
+
+
+def test_equivalent_sources_small_data(hm, synthetic):
+    """test_eq_sources_cartesian.py:155-198 (damping None, depth 500)."""
+    points, masses, _, _ = synthetic
+    coords = grid_coordinates(REGION, shape=(8, 8), extra=0)
+    data = oracle.point_gravity(coords, points, masses, "g_z")
+    eqs = hm.EquivalentSources(depth=500).fit(coords, data)
+    npt.assert_allclose(data, eqs.predict(coords), rtol=1e-5)
+    tmp = [c.ravel() for c in coords]
+    npt.assert_allclose(tmp[:2], eqs.points_[:2], rtol=1e-5)
+    npt.assert_allclose(tmp[2] - 500, eqs.points_[2], rtol=1e-5)
+    grid_coords = grid_coordinates(REGION, shape=(8, 8), extra=20)
+    true = oracle.point_gravity(grid_coords, points, masses, "g_z")
+    npt.assert_allclose(true, eqs.predict(grid_coords), rtol=0.08)
+    npt.assert_allclose(true, eqs.grid(grid_coords).scalars, rtol=0.08)
+    profile = eqs.profile((REGION[0], REGION[2]), (REGION[0], REGION[3]), 20, 10)
+    true = oracle.point_gravity((profile.easting, profile.northing, profile.upward), points,
+                                masses, "g_z")
+    npt.assert_allclose(true, profile.scalars, rtol=0.05)
+
+
+def test_custom_points_and_depth(hm, synthetic):
+    """test_eq_sources_cartesian.py:255-281."""
+    _, _, coords, data = synthetic
+    custom = tuple(c.ravel() for c in grid_coordinates(REGION, shape=(3, 3), extra=-550))
+    eqs = hm.EquivalentSources(points=custom).fit(coords, data)
+    npt.assert_allclose(custom, eqs.points_, rtol=1e-5)
+    assert eqs.depth_ is None
+    small = grid_coordinates(REGION, shape=(5, 5))
+    small = (small[0], small[1], np.arange(25, dtype=float).reshape(5, 5))
+    d_small = oracle.point_gravity(small, custom, np.ones(9) * 1e9, "g_z")
+    eqs = hm.EquivalentSources(depth=1.3e3).fit(small, d_small)
+    npt.assert_allclose((small[0].ravel(), small[1].ravel(), small[2].ravel() - 1.3e3), eqs.points_)
+
+
+@pytest.mark.parametrize("dtype", ["float64", "float32"])
+@pytest.mark.parametrize("block_size", [None, 500])
+@pytest.mark.parametrize("weights_none", [False, True])
+def test_dtype(hm, synthetic, dtype, block_size, weights_none):
+    """test_eq_sources_cartesian.py:339-378 (subset of the parameter grid)."""
+    _, _, coords, data = synthetic
+    weights = None if weights_none else np.ones_like(data)
+    eqs = hm.EquivalentSources(damping=0.1, block_size=block_size, dtype=dtype)
+    eqs.fit(coords, data, weights)
+    prediction = eqs.predict(coords)
+    for c in eqs.points_:
+        assert c.dtype == np.dtype(dtype)
+    assert prediction.dtype == np.dtype(dtype)
+
+
+def test_grid_argument_errors(hm, synthetic):
+    """test_eq_sources_cartesian.py:407-445."""
+    _, _, coords, data = synthetic
+    eqs = hm.EquivalentSources(damping=0.1).fit(coords, data)
+    grid_coords = grid_coordinates(REGION, shape=(4, 4), extra=2e3)
+    with pytest.raises(ValueError, match="The 'upward', 'region', 'shape' and 'spacing'"):
+        eqs.grid(coordinates=grid_coords, upward=5e3, spacing=1)
+    with pytest.warns(FutureWarning, match="The 'bla' arguments are being ignored."):
+        eqs.grid(coordinates=grid_coords, bla="bla")
+
+
+def test_default_depth_fit(hm, synthetic):
+    _, _, coords, data = synthetic
+    eqs = hm.EquivalentSources(damping=0.1).fit(coords, data)
+    spacing = min(coords[0][1, 1] - coords[0][0, 0], coords[1][1, 1] - coords[1][0, 0])
+    npt.assert_allclose(eqs.depth_, 4.5 * spacing)
+
+
+def test_mixed_precision_fit_matches_fp64(hm):
+    """FP32-MFMA preconditioned fit against the FP64 fit: predictions on a
+    grid agree to 1e-6 of their maximum."""
+    rng = np.random.default_rng(9)
+    n = 3000
+    L = np.sqrt(n * 0.16e6)
+    coords = (rng.uniform(0, L, n), rng.uniform(0, L, n), rng.uniform(500, 1500, n))
+    prisms = np.array([[0.3 * L, 0.5 * L, 0.2 * L, 0.6 * L, -4000.0, -1500.0]])
+    data = oracle.prism_gravity(coords, prisms, [300.0], "g_z") + rng.normal(0, 0.05, n)
+    grid = grid_coordinates((0, L, 0, L), shape=(40, 40), extra=1000.0)
+    fp64 = hm.EquivalentSources(damping=1e-3, depth=4500).fit(coords, data)
+    mixed = hm.EquivalentSources(damping=1e-3, depth=4500, fit_precision="mixed").fit(coords, data)
+    p64, pmx = fp64.predict(grid), mixed.predict(grid)
+    err = np.abs(pmx - p64).max() / np.abs(p64).max()
+    assert err <= 1e-6, (err, mixed.fit_info_)
