@@ -57,6 +57,12 @@ SIGNATURES = {
     "hmx_eqs_fit_workspace_bytes": (_int, [_i64, _i64, _int, _szp]),
     "hmx_eqs_fit": (_int, [_vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _i64, _dbl, _int, _int,
                            _dbl, _vp, ctypes.POINTER(ctypes.c_double), _vp, ctypes.c_size_t, _vp]),
+    "hmx_mfma_probe": (_int, [_int, _vp, _vp, _vp, _vp]),
+    "hmx_gemm_tn": (_int, [_int, _vp, _i64, _vp, _i64, _i64, _i64, _i64, _vp, _i64, _dbl, _dbl,
+                           _dbl, _int, _vp]),
+    "hmx_cholesky_workspace_bytes": (_int, [_i64, _szp]),
+    "hmx_cholesky_solve": (_int, [_vp, _i64, _vp, ctypes.POINTER(ctypes.c_int), _vp,
+                                  ctypes.c_size_t, _vp]),
     "hmx_devmath_eval": (_int, [_int, _vp, _vp, _vp, _i64, _vp]),
     "hmx_eqs_predict_workspace_bytes": (_int, [_i64, _i64, _szp]),
     "hmx_eqs_predict": (_int, [_vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _i64, _int, _vp, _vp,

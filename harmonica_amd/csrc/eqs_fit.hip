@@ -107,12 +107,17 @@ __global__ __launch_bounds__(256) void scaled_jacobian_kernel(
 // -------------------------------------------------------------- MFMA SYRK
 template <class T>
 struct Mfma;
+// Operand layout of both 16x16x4 MFMAs (A 16x4, B 4x16): lane l supplies
+// A[l % 16][l / 16] and B[l / 16][l % 16].  Element v of lane l's result is
+// D[row(l, v)][l % 16] with row = l / 16 + 4 v for f64 and 4 (l / 16) + v
+// for f32 (checked by tests/test_gpu_linalg.py through hmx_mfma_probe).
 template <>
 struct Mfma<double> {
     typedef double4_t acc_t;
     static __device__ __forceinline__ acc_t step(double a, double b, acc_t c) {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
+    static __device__ __forceinline__ int row(int lane, int v) { return (lane >> 4) + 4 * v; }
 };
 template <>
 struct Mfma<float> {
@@ -120,6 +125,7 @@ struct Mfma<float> {
     static __device__ __forceinline__ acc_t step(float a, float b, acc_t c) {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
+    static __device__ __forceinline__ int row(int lane, int v) { return 4 * (lane >> 4) + v; }
 };
 
 // C = beta * C + sign * A^T B + shift * I on the selected tiles, with A
@@ -131,10 +137,6 @@ struct Mfma<float> {
 // chunks of kKC rows.  Every workgroup reads all its inputs before it writes
 // its tile, and tiles are disjoint, so C may alias B when M <= kST (used by
 // the in-place panel solve of the Cholesky factorisation).
-//
-// Operand layout of the 16x16x4 MFMA (A 16x4, B 4x16, D 16x16): lane l
-// supplies A[l % 16][l / 16] and B[l / 16][l % 16]; element v of lane l's
-// accumulator is D[4 (l / 16) + v][l % 16].
 template <class T, bool UPPER>
 __global__ __launch_bounds__(256) void gemm_tn_kernel(const T *__restrict__ A, int64_t lda,
                                                       const T *B, int64_t ldb, int64_t K,
@@ -199,7 +201,7 @@ __global__ __launch_bounds__(256) void gemm_tn_kernel(const T *__restrict__ A, i
         for (int tj = 0; tj < 4; ++tj)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int64_t r = r0 + 64 * wr + 16 * ti + 4 * (lane >> 4) + v;
+                const int64_t r = r0 + 64 * wr + 16 * ti + Mfma<T>::row(lane, v);
                 const int64_t c = c0 + 64 * wc + 16 * tj + (lane & 15);
                 if (r < M && c < N && (!UPPER || r <= c)) {
                     double val = sign * (double)acc[ti][tj][v];
@@ -592,10 +594,115 @@ struct Fit {
     }
 };
 
+// One 16x16x4 MFMA of one wavefront with per-lane operands a[l], b[l];
+// out[4 l + v] = element v of lane l's result (layout check).
+__global__ void mfma_probe_kernel(int dtype, const double *__restrict__ a,
+                                  const double *__restrict__ b, double *__restrict__ out) {
+    const int l = threadIdx.x;
+    if (dtype == HMX_F64) {
+        double4_t c = {0, 0, 0, 0};
+        c = Mfma<double>::step(a[l], b[l], c);
+        for (int v = 0; v < 4; ++v) out[4 * l + v] = c[v];
+    } else {
+        float4_t c = {0, 0, 0, 0};
+        c = Mfma<float>::step((float)a[l], (float)b[l], c);
+        for (int v = 0; v < 4; ++v) out[4 * l + v] = (double)c[v];
+    }
+}
+
 }  // namespace
 }  // namespace hmx
 
 extern "C" {
+
+int hmx_mfma_probe(int dtype, const double *a, const double *b, double *out, void *stream) {
+    using namespace hmx;
+    clear_error();
+    if (dtype != HMX_F64 && dtype != HMX_F32) return fail(HMX_ERR_ARG, "invalid dtype %d", dtype);
+    if (!a || !b || !out) return fail(HMX_ERR_ARG, "NULL array");
+    hipLaunchKernelGGL(mfma_probe_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, dtype, a, b, out);
+    HMX_LAUNCHED("mfma_probe_kernel");
+    return HMX_OK;
+}
+
+int hmx_gemm_tn(int dtype, const void *A, int64_t lda, const void *B, int64_t ldb, int64_t K,
+                int64_t M, int64_t N, double *C, int64_t ldc, double beta, double sign,
+                double shift, int upper, void *stream) {
+    using namespace hmx;
+    clear_error();
+    if (K < 0 || M < 0 || N < 0) return fail(HMX_ERR_ARG, "negative size");
+    if (lda < M || ldb < N || ldc < N) return fail(HMX_ERR_ARG, "leading dimension too small");
+    if (upper && M != N) return fail(HMX_ERR_ARG, "upper requires M == N");
+    if (M == 0 || N == 0) return HMX_OK;
+    if (!A || !B || !C) return fail(HMX_ERR_ARG, "NULL array");
+    hipStream_t s = (hipStream_t)stream;
+    const int64_t tm = (M + kST - 1) / kST, tn = (N + kST - 1) / kST;
+    const unsigned blocks = (unsigned)(upper ? tn * (tn + 1) / 2 : tm * tn);
+    if (dtype == HMX_F64) {
+        if (upper)
+            hipLaunchKernelGGL((gemm_tn_kernel<double, true>), dim3(blocks), dim3(256), 0, s,
+                               (const double *)A, lda, (const double *)B, ldb, K, M, N, C, ldc,
+                               beta, sign, shift, tn);
+        else
+            hipLaunchKernelGGL((gemm_tn_kernel<double, false>), dim3(blocks), dim3(256), 0, s,
+                               (const double *)A, lda, (const double *)B, ldb, K, M, N, C, ldc,
+                               beta, sign, shift, tn);
+    } else if (dtype == HMX_F32) {
+        if (upper)
+            hipLaunchKernelGGL((gemm_tn_kernel<float, true>), dim3(blocks), dim3(256), 0, s,
+                               (const float *)A, lda, (const float *)B, ldb, K, M, N, C, ldc, beta,
+                               sign, shift, tn);
+        else
+            hipLaunchKernelGGL((gemm_tn_kernel<float, false>), dim3(blocks), dim3(256), 0, s,
+                               (const float *)A, lda, (const float *)B, ldb, K, M, N, C, ldc, beta,
+                               sign, shift, tn);
+    } else {
+        return fail(HMX_ERR_ARG, "invalid dtype %d", dtype);
+    }
+    HMX_LAUNCHED("gemm_tn_kernel");
+    return HMX_OK;
+}
+
+int hmx_cholesky_workspace_bytes(int64_t m, size_t *bytes) {
+    using namespace hmx;
+    if (!bytes) return fail(HMX_ERR_ARG, "bytes is NULL");
+    if (m <= 0) return fail(HMX_ERR_ARG, "empty matrix");
+    const size_t nblk = (size_t)((m + kNB - 1) / kNB);
+    *bytes = 2 * nblk * kNB * kNB * sizeof(double) + (2 * (size_t)m + 256) * sizeof(double) + 256;
+    return HMX_OK;
+}
+
+int hmx_cholesky_solve(double *A, int64_t m, double *x, int *info, void *workspace,
+                       size_t workspace_bytes, void *stream) {
+    using namespace hmx;
+    clear_error();
+    size_t need = 0;
+    int st = hmx_cholesky_workspace_bytes(m, &need);
+    if (st) return st;
+    if (!A || !x || !info) return fail(HMX_ERR_ARG, "NULL array");
+    if (!workspace || workspace_bytes < need)
+        return fail(HMX_ERR_ARG, "workspace too small: %zu < %zu bytes", workspace_bytes, need);
+    const size_t nblk = (size_t)((m + kNB - 1) / kNB);
+    Fit f;
+    f.m = m;
+    f.s = (hipStream_t)stream;
+    f.A = A;
+    f.Dinv = static_cast<double *>(workspace);
+    f.DinvT = f.Dinv + nblk * kNB * kNB;
+    double *wk = f.DinvT + nblk * kNB * kNB;
+    double *wk2 = wk + m;
+    f.part = wk2 + m;
+    f.dinfo = reinterpret_cast<int *>(f.part + 256);
+    st = f.cholesky(info);
+    if (st) return st;
+    if (*info) return HMX_OK;
+    hipLaunchKernelGGL(copy_kernel, dim3(blocks_for(m)), dim3(256), 0, f.s, m, x, wk);
+    HMX_LAUNCHED("copy_kernel");
+    st = f.solve(wk, wk2, x);
+    if (st) return st;
+    HMX_HIP(hipStreamSynchronize(f.s));
+    return HMX_OK;
+}
 
 int hmx_eqs_fit_workspace_bytes(int64_t n_data, int64_t n_src, int precision_mode,
                                 size_t *bytes) {

@@ -186,6 +186,25 @@ int hmx_eqs_fit(const double *easting, const double *northing, const double *upw
                 double *coefs, double *info, void *workspace, size_t workspace_bytes,
                 void *stream);
 
+/* Building blocks of the fit, exposed for testing and reuse (no
+ * counterpart in the reference):
+ * hmx_mfma_probe: one 16x16x4 MFMA (dtype HMX_F64 or HMX_F32) of a single
+ *   wavefront with lane operands a[l], b[l] (64 each); out[4 l + v] is
+ *   element v of lane l's accumulator (documents the operand layout).
+ * hmx_gemm_tn: C = beta C + sign A^T B + shift I on MFMA, A (K x M, lda),
+ *   B (K x N, ldb) row-major in dtype, C (M x N, ldc) FP64; upper != 0
+ *   computes only the upper triangle (M == N).
+ * hmx_cholesky_solve: factor the SPD matrix A (m x m, row-major, upper
+ *   triangle read) in place as U^T U (FP64, blocked) and overwrite x with
+ *   A^-1 x; *info (host) = 0 or the 1-based column of a failed pivot. */
+int hmx_mfma_probe(int dtype, const double *a, const double *b, double *out, void *stream);
+int hmx_gemm_tn(int dtype, const void *A, int64_t lda, const void *B, int64_t ldb, int64_t K,
+                int64_t M, int64_t N, double *C, int64_t ldc, double beta, double sign,
+                double shift, int upper, void *stream);
+int hmx_cholesky_workspace_bytes(int64_t m, size_t *bytes);
+int hmx_cholesky_solve(double *A, int64_t m, double *x, int *info, void *workspace,
+                       size_t workspace_bytes, void *stream);
+
 /* Device math routines exposed for accuracy testing (no counterpart in the
  * reference; these are the FP64 sqrt, division, logarithm and
  * principal-value arctangent used inside the pairwise kernels):

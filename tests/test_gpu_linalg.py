@@ -1,0 +1,164 @@
+"""
+The MFMA and Cholesky building blocks of the equivalent-source fit:
+operand/result lane layout of the 16x16x4 MFMA builtins, the MFMA GEMM
+(C = beta C + sign A^T B + shift I, full and upper-triangular) against
+numpy, and the blocked FP64 Cholesky solve against numpy.
+"""
+
+import ctypes
+
+import numpy as np
+import numpy.testing as npt
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def env():
+    import torch
+
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from harmonica_amd import _lib
+
+    return torch, _lib, _lib.lib()
+
+
+def _layouts(a, b, dtype):
+    """Result D (16 x 16) of one MFMA under the layout the kernels assume:
+    lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16]; element v of
+    lane l holds D[row][l % 16] with row = l / 16 + 4 v (f64) or
+    4 (l / 16) + v (f32)."""
+    A = np.zeros((16, 4))
+    B = np.zeros((4, 16))
+    for l in range(64):
+        A[l % 16, l // 16] = a[l]
+        B[l // 16, l % 16] = b[l]
+    D = A @ B
+    out = np.zeros(256)
+    for l in range(64):
+        for v in range(4):
+            row = l // 16 + 4 * v if dtype == 0 else 4 * (l // 16) + v
+            out[4 * l + v] = D[row, l % 16]
+    return out
+
+
+@pytest.mark.parametrize("dtype", [0, 1], ids=["f64", "f32"])
+def test_mfma_layout(env, dtype):
+    torch, _lib, L = env
+    rng = np.random.default_rng(0)
+    a = rng.integers(-8, 9, 64).astype(float)
+    b = rng.integers(-8, 9, 64).astype(float)
+    A, B = torch.tensor(a).cuda(), torch.tensor(b).cuda()
+    out = torch.empty(256, dtype=torch.float64, device="cuda")
+    _lib.check(L.hmx_mfma_probe(dtype, _lib.ptr(A), _lib.ptr(B), _lib.ptr(out), None))
+    got = out.cpu().numpy()
+    expected = _layouts(a, b, dtype)
+    assert np.array_equal(got, expected), f"matching layouts: {_which(a, b, got)}"
+
+
+def _which(a, b, got):
+    """Diagnose: which (A map, B map, D map) combinations reproduce `got`."""
+    amaps = {"A[l%16][l/16]": lambda l: (l % 16, l // 16), "A[l/4][l%4]": lambda l: (l // 4, l % 4)}
+    bmaps = {"B[l/16][l%16]": lambda l: (l // 16, l % 16), "B[l%4][l/4]": lambda l: (l % 4, l // 4)}
+    dmaps = {
+        "D[4(l/16)+v][l%16]": lambda l, v: (4 * (l // 16) + v, l % 16),
+        "D[l%16][4(l/16)+v]": lambda l, v: (l % 16, 4 * (l // 16) + v),
+        "D[(l/16)+4v][l%16]": lambda l, v: (l // 16 + 4 * v, l % 16),
+    }
+    found = []
+    for an, am in amaps.items():
+        for bn, bm in bmaps.items():
+            A = np.zeros((16, 4))
+            B = np.zeros((4, 16))
+            for l in range(64):
+                A[am(l)] = a[l]
+                B[bm(l)] = b[l]
+            D = A @ B
+            for dn, dmap in dmaps.items():
+                out = np.array([D[dmap(l, v)] for l in range(64) for v in range(4)])
+                if np.array_equal(out, got):
+                    found.append((an, bn, dn))
+    return found
+
+
+def _gemm(env, dtype, A, B, C, beta, sign, shift, upper):
+    torch, _lib, L = env
+    tdt = torch.float64 if dtype == 0 else torch.float32
+    dA = torch.tensor(A, dtype=tdt).cuda()
+    dB = torch.tensor(B, dtype=tdt).cuda()
+    dC = torch.tensor(C, dtype=torch.float64).cuda()
+    K, M = A.shape
+    N = B.shape[1]
+    _lib.check(L.hmx_gemm_tn(dtype, _lib.ptr(dA), M, _lib.ptr(dB), N, K, M, N, _lib.ptr(dC), N,
+                             beta, sign, shift, int(upper), None))
+    return dC.cpu().numpy()
+
+
+@pytest.mark.parametrize("shape", [(37, 19, 23), (200, 130, 260), (16, 128, 128), (5, 300, 7)])
+@pytest.mark.parametrize("dtype", [0, 1], ids=["f64", "f32"])
+def test_gemm_tn_full(env, shape, dtype):
+    K, M, N = shape
+    rng = np.random.default_rng(K + M + N)
+    A, B, C = rng.normal(size=(K, M)), rng.normal(size=(K, N)), rng.normal(size=(M, N))
+    if dtype == 1:
+        A, B = A.astype(np.float32).astype(float), B.astype(np.float32).astype(float)
+    got = _gemm(env, dtype, A, B, C, 0.5, -1.0, 0.0, False)
+    expected = 0.5 * C - A.T @ B
+    tol = 1e-12 if dtype == 0 else 1e-5
+    npt.assert_allclose(got, expected, atol=tol * np.abs(expected).max())
+
+
+@pytest.mark.parametrize("dtype", [0, 1], ids=["f64", "f32"])
+@pytest.mark.parametrize("m", [1, 77, 128, 300])
+def test_syrk_upper(env, dtype, m):
+    rng = np.random.default_rng(m)
+    X = rng.normal(size=(150, m))
+    if dtype == 1:
+        X = X.astype(np.float32).astype(float)
+    C = np.full((m, m), 7.0)
+    got = _gemm(env, dtype, X, X, C, 0.0, 1.0, 2.5, True)
+    expected = X.T @ X + 2.5 * np.eye(m)
+    iu = np.triu_indices(m)
+    tol = 1e-12 if dtype == 0 else 1e-5
+    npt.assert_allclose(got[iu], expected[iu], atol=tol * np.abs(expected).max())
+    il = np.tril_indices(m, -1)
+    assert np.all(got[il] == 7.0)  # lower triangle untouched
+
+
+@pytest.mark.parametrize("m", [5, 128, 129, 300, 700])
+def test_cholesky_solve(env, m):
+    torch, _lib, L = env
+    rng = np.random.default_rng(m)
+    X = rng.normal(size=(m + 20, m))
+    A = X.T @ X + 0.5 * np.eye(m)
+    rhs = rng.normal(size=m)
+    nbytes = ctypes.c_size_t(0)
+    _lib.check(L.hmx_cholesky_workspace_bytes(m, ctypes.byref(nbytes)))
+    ws = torch.empty(nbytes.value, dtype=torch.uint8, device="cuda")
+    dA = torch.tensor(A).cuda()
+    dx = torch.tensor(rhs).cuda()
+    info = ctypes.c_int(-1)
+    _lib.check(L.hmx_cholesky_solve(_lib.ptr(dA), m, _lib.ptr(dx), ctypes.byref(info), _lib.ptr(ws),
+                                    nbytes.value, None))
+    assert info.value == 0
+    expected = np.linalg.solve(A, rhs)
+    npt.assert_allclose(dx.cpu().numpy(), expected, rtol=1e-9, atol=1e-11 * np.abs(expected).max())
+    U = np.triu(dA.cpu().numpy())
+    npt.assert_allclose(U.T @ U, A, atol=1e-10 * np.abs(A).max())
+
+
+def test_cholesky_reports_indefinite(env):
+    torch, _lib, L = env
+    m = 200
+    A = np.eye(m)
+    A[150, 150] = -1.0
+    nbytes = ctypes.c_size_t(0)
+    _lib.check(L.hmx_cholesky_workspace_bytes(m, ctypes.byref(nbytes)))
+    ws = torch.empty(nbytes.value, dtype=torch.uint8, device="cuda")
+    dA, dx = torch.tensor(A).cuda(), torch.ones(m, dtype=torch.float64, device="cuda")
+    info = ctypes.c_int(-1)
+    _lib.check(L.hmx_cholesky_solve(_lib.ptr(dA), m, _lib.ptr(dx), ctypes.byref(info), _lib.ptr(ws),
+                                    nbytes.value, None))
+    assert info.value == 151
