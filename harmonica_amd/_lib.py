@@ -57,6 +57,15 @@ SIGNATURES = {
     "hmx_eqs_fit_workspace_bytes": (_int, [_i64, _i64, _int, _szp]),
     "hmx_eqs_fit": (_int, [_vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _i64, _dbl, _int, _int,
                            _dbl, _vp, ctypes.POINTER(ctypes.c_double), _vp, ctypes.c_size_t, _vp]),
+    "hmx_gz_grad_workspace_bytes": (_int, [_i64, _i64, _szp]),
+    "hmx_prism_gz_jvp": (_int, [_vp, _vp, _vp, _i64, _vp, _vp, _vp, _i64, _vp, _vp, ctypes.c_size_t,
+                                _vp]),
+    "hmx_prism_gz_vjp": (_int, [_vp, _vp, _vp, _vp, _i64, _vp, _i64, _vp, _vp, _vp, ctypes.c_size_t,
+                                _vp]),
+    "hmx_prism_layer_gz_jvp": (_int, [_vp, _vp, _vp, _i64, _vp, _i64, _vp, _i64, _vp, _vp, _vp, _dbl,
+                                      _vp, _vp, _vp, _vp, ctypes.c_size_t, _vp]),
+    "hmx_prism_layer_gz_vjp": (_int, [_vp, _vp, _vp, _vp, _i64, _vp, _i64, _vp, _i64, _vp, _vp, _vp,
+                                      _dbl, _vp, _vp, _vp, ctypes.c_size_t, _vp]),
     "hmx_mfma_probe": (_int, [_int, _vp, _vp, _vp, _vp]),
     "hmx_gemm_tn": (_int, [_int, _vp, _i64, _vp, _i64, _i64, _i64, _i64, _vp, _i64, _dbl, _dbl,
                            _dbl, _int, _vp]),

@@ -143,6 +143,44 @@ int hmx_eqs_jacobian(const void *easting, const void *northing, const void *upwa
                      const void *p_upward, int64_t n_src, int dtype, void *jac, int64_t ld,
                      void *stream);
 
+/* Derivatives of g_z with respect to prism densities and top heights (no
+ * counterpart in the reference, whose forward models are not
+ * differentiable; these serve the autograd path and Gauss-Newton
+ * inversion).  With S(i, p) the g_u vertex sum of prism p at point i
+ * (gravity.py closed form without G rho) and F(i, p) the signed sum over
+ * the 4 top vertices of atan(x y / (z r)):
+ *   d g_u(i) / d rho_p = G S(i, p),  d g_u(i) / d top_p = G rho_p F(i, p).
+ * jvp: out_i = sum_p weight_s[p] S(i, p) + weight_f[p] F(i, p)
+ *      (weight_f may be NULL);
+ * vjp: grad_s[p] = sum_i v_i S(i, p), grad_f[p] = sum_i v_i F(i, p).
+ * Raw sums: the caller applies G and harmonica's -1e5 for g_z in mGal.
+ * Layer variants take the layer arrays of hmx_prism_layer_gravity; the
+ * per-prism weights and gradients are (n_north, n_east) arrays; prisms
+ * skipped by the layer rules (NaN density, top or bottom, thickness below
+ * the threshold) contribute nothing, zero-density prisms are kept (the
+ * derivative with respect to a zero density is not zero). */
+int hmx_gz_grad_workspace_bytes(int64_t n_obs, int64_t n_prisms, size_t *bytes);
+int hmx_prism_gz_jvp(const double *easting, const double *northing, const double *upward,
+                     int64_t n_obs, const double *prisms, const double *weight_s,
+                     const double *weight_f, int64_t n_prisms, double *out, void *workspace,
+                     size_t workspace_bytes, void *stream);
+int hmx_prism_gz_vjp(const double *easting, const double *northing, const double *upward,
+                     const double *v, int64_t n_obs, const double *prisms, int64_t n_prisms,
+                     double *grad_s, double *grad_f, void *workspace, size_t workspace_bytes,
+                     void *stream);
+int hmx_prism_layer_gz_jvp(const double *easting, const double *northing, const double *upward,
+                           int64_t n_obs, const double *centre_easting, int64_t n_east,
+                           const double *centre_northing, int64_t n_north, const double *bottom,
+                           const double *top, const double *density, double thickness_threshold,
+                           const double *weight_s, const double *weight_f, double *out,
+                           void *workspace, size_t workspace_bytes, void *stream);
+int hmx_prism_layer_gz_vjp(const double *easting, const double *northing, const double *upward,
+                           const double *v, int64_t n_obs, const double *centre_easting,
+                           int64_t n_east, const double *centre_northing, int64_t n_north,
+                           const double *bottom, const double *top, const double *density,
+                           double thickness_threshold, double *grad_s, double *grad_f,
+                           void *workspace, size_t workspace_bytes, void *stream);
+
 /* Bytes of workspace needed by hmx_eqs_predict on the current device. */
 int hmx_eqs_predict_workspace_bytes(int64_t n_obs, int64_t n_src, size_t *bytes);
 
