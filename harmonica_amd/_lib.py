@@ -66,6 +66,12 @@ SIGNATURES = {
                                       _vp, _vp, _vp, _vp, ctypes.c_size_t, _vp]),
     "hmx_prism_layer_gz_vjp": (_int, [_vp, _vp, _vp, _vp, _i64, _vp, _i64, _vp, _i64, _vp, _vp, _vp,
                                       _dbl, _vp, _vp, _vp, ctypes.c_size_t, _vp]),
+    "hmx_batched_plan_size": (_int, [_vp, _vp, _i64, _i64, ctypes.POINTER(ctypes.c_int64)]),
+    "hmx_batched_plan": (_int, [_vp, _vp, _i64, _i64, _vp, _i64, ctypes.POINTER(ctypes.c_int64), _vp,
+                                _vp]),
+    "hmx_batched_workspace_bytes": (_int, [_i64, _szp]),
+    "hmx_batched_prism_gz": (_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _i64, _i64,
+                                    _int, _vp, _vp, ctypes.c_size_t, _vp]),
     "hmx_mfma_probe": (_int, [_int, _vp, _vp, _vp, _vp]),
     "hmx_gemm_tn": (_int, [_int, _vp, _i64, _vp, _i64, _i64, _i64, _i64, _vp, _i64, _dbl, _dbl,
                            _dbl, _int, _vp]),

@@ -181,6 +181,33 @@ int hmx_prism_layer_gz_vjp(const double *easting, const double *northing, const 
                            double thickness_threshold, double *grad_s, double *grad_f,
                            void *workspace, size_t workspace_bytes, void *stream);
 
+/* Many independent g_z prism models in one persistent launch (no
+ * counterpart in the reference, where each model is one call of
+ * jit_prism_gravity, gravity.py:489-545).  Model b owns points
+ * [obs_offsets[b], obs_offsets[b+1]) of the concatenated point arrays and
+ * prisms [prism_offsets[b], prism_offsets[b+1]) of the concatenated
+ * (n, 6) prism array.
+ * hmx_batched_plan_size / hmx_batched_plan (HOST arrays): cut every model
+ *   into items (model, 256-point tile, chunk of <= prisms_per_item prisms,
+ *   slot), 5 int64 each, in decreasing-cost order; item_base[b] is the
+ *   first slot of model b and chunks[b] its chunk count.
+ * hmx_batched_prism_gz (DEVICE arrays): runs the items with a persistent
+ *   grid of blocks_per_cu workgroups per compute unit pulling from an
+ *   atomic counter, then writes out (n_obs_total) = g_z in mGal.  Only
+ *   kernel launches (graph-capturable). */
+int hmx_batched_plan_size(const int64_t *obs_offsets, const int64_t *prism_offsets,
+                          int64_t n_models, int64_t prisms_per_item, int64_t *n_items);
+int hmx_batched_plan(const int64_t *obs_offsets, const int64_t *prism_offsets, int64_t n_models,
+                     int64_t prisms_per_item, int64_t *items, int64_t capacity, int64_t *n_items,
+                     int64_t *item_base, int64_t *chunks);
+int hmx_batched_workspace_bytes(int64_t n_items, size_t *bytes);
+int hmx_batched_prism_gz(const double *easting, const double *northing, const double *upward,
+                         const int64_t *obs_offsets, const double *prisms, const double *density,
+                         const int64_t *items, int64_t n_items, const int64_t *item_base,
+                         const int64_t *chunks, int64_t n_models, int64_t n_obs_total,
+                         int blocks_per_cu, double *out, void *workspace, size_t workspace_bytes,
+                         void *stream);
+
 /* Bytes of workspace needed by hmx_eqs_predict on the current device. */
 int hmx_eqs_predict_workspace_bytes(int64_t n_obs, int64_t n_src, size_t *bytes);
 

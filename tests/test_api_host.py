@@ -203,3 +203,30 @@ def test_workloads_are_seeded():
     assert a["coordinates"][0].shape == (64, 64)
     big = workloads.terrain_layer_large(n_prisms_side=100, n_obs_side=8)
     assert big["surface"].min() == 0.0 and big["surface"].max() == 2000.0
+
+
+def test_batched_plan_on_host():
+    """The work-list planner runs on the host: every (tile, chunk) slot once,
+    longest items first, per-model slot bases and chunk counts."""
+    obs_off = np.array([0, 300, 300, 1000], dtype=np.int64)
+    pr_off = np.array([0, 5000, 6000, 6100], dtype=np.int64)
+    L = _lib.lib()
+    n = ctypes.c_int64(0)
+    assert L.hmx_batched_plan_size(obs_off.ctypes.data_as(ctypes.c_void_p),
+                                   pr_off.ctypes.data_as(ctypes.c_void_p), 3, 2048,
+                                   ctypes.byref(n)) == 0
+    # model 0: 2 tiles x 3 chunks, model 1: no points, model 2: 3 tiles x 1 chunk
+    assert n.value == 2 * 3 + 0 + 3
+    items = np.zeros(5 * n.value, dtype=np.int64)
+    base, chunks = np.zeros(3, dtype=np.int64), np.zeros(3, dtype=np.int64)
+    got = ctypes.c_int64(0)
+    assert L.hmx_batched_plan(obs_off.ctypes.data_as(ctypes.c_void_p),
+                              pr_off.ctypes.data_as(ctypes.c_void_p), 3, 2048,
+                              items.ctypes.data_as(ctypes.c_void_p), n.value, ctypes.byref(got),
+                              base.ctypes.data_as(ctypes.c_void_p),
+                              chunks.ctypes.data_as(ctypes.c_void_p)) == 0
+    it = items.reshape(-1, 5)
+    assert sorted(it[:, 4]) == list(range(n.value))
+    cost = it[:, 3] - it[:, 2]
+    assert np.all(np.diff(cost) <= 0)
+    assert list(base) == [0, 6, 6] and list(chunks) == [3, 1, 1]
