@@ -28,12 +28,20 @@ sys.path.insert(0, ROOT)
 # MI355X FP64 vector peak (AMD published: 256 CU x 128 FP64 flop/clk x 2.4 GHz).
 FP64_PEAK = 78.6e12
 
-# Algorithmic FP64 operations per prism-observation pair of the g_z closed
-# form, frozen from rocprofv3 FP64 VALU counters of the first (ocml-based)
-# kernel version: 8 vertices x (sqrt + 2 log + atan + division + the
-# arithmetic of the kernel), FMA = 2.  Kept fixed so later, leaner
-# implementations are credited in pair-evals/s, not in flop count.
-FLOPS_PER_PAIR = {"g_z": None, "all": None}
+# FP64 operations per prism-observation pair of the g_z kernel
+# (prism_forward_kernel<g_z>): rocprofv3 counters SQ_INSTS_VALU_{ADD,MUL,
+# TRANS}_F64 + 2 x SQ_INSTS_VALU_FMA_F64, x 64 lanes, divided by the pairs of
+# the profiled launch (profiles/prism_gz_terrain1M_counters.md).  The same
+# profile gives 999.4 VALU instructions per pair; every wave64 VALU
+# instruction occupies a SIMD for 4 cycles, which bounds the pair rate at
+# 1024 SIMDs x 2.4 GHz x 16 / 999.4.
+FLOPS_PER_PAIR_GZ = 1083.7
+VALU_INSTR_PER_PAIR_GZ = 999.4
+VALU_ISSUE_RATE = 1024 * 2.4e9 * 16  # lane-instructions per second, wave64 at 4 cycles
+# HBM bytes per launch of the g_z kernel on this workload: FETCH_SIZE
+# 220218.8 KiB x 2 (a 1 GiB copy reads as 524308 KiB) + WRITE_SIZE 20480 KiB
+# x 1 (profiles/prism_gz_terrain1M_counters.md).
+TRAFFIC_BYTES_GZ = (220218.8 * 2 + 20480.0) * 1024
 
 
 def log(msg):
@@ -59,6 +67,156 @@ def cpu_baseline(w, prisms, density):
             "host_cores_visible": cores}
 
 
+def _sync_time(fn):
+    import torch
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = fn()
+    torch.cuda.synchronize()
+    return res, time.perf_counter() - t0
+
+
+def bench_layer_fused(args):
+    """Ten fields of the 1000 x 1000 terrain layer in one fused launch
+    through the prism-layer accessor path (hmx_prism_layer_gravity)."""
+    import torch
+
+    from harmonica_amd._forward import layer_fields_device, to_device
+    from harmonica_amd._lib import FIELDS
+    from harmonica_amd.workloads import terrain_layer_large
+
+    w = terrain_layer_large(n_prisms_side=args.prism_side, n_obs_side=args.obs_side)
+    E, N, U = (to_device(c.ravel()) for c in w["coordinates"])
+    top = np.maximum(w["surface"], w["reference"])
+    bottom = np.minimum(w["surface"], w["reference"]) + 0 * top
+    args_dev = (to_device(w["easting"]), to_device(w["northing"]), to_device(bottom),
+                to_device(top), to_device(w["density"]))
+    pairs = E.numel() * w["density"].size
+    out = {}
+    for name, fields in (("g_z", ("g_z",)), ("all_ten", FIELDS)):
+        layer_fields_device(E, N, U, *args_dev, fields)
+        _, dt = _sync_time(lambda: layer_fields_device(E, N, U, *args_dev, fields))
+        out[name] = {"s": dt, "pair_evals_per_s": pairs / dt}
+        log(f"layer {name}: {dt:.3f} s, {pairs / dt:.4e} pair-evals/s")
+    return out
+
+
+def bench_batched(args):
+    """512 independent uneven models (log-uniform 1e3-2e5 prisms, 1e3-5e4
+    points), g_z, one persistent graph-captured launch, autotuned."""
+    from harmonica_amd._batched import BatchedPrismGz
+    from harmonica_amd.workloads import batched_models
+
+    models = batched_models(n_models=args.batched_models)
+    batch = BatchedPrismGz(models)
+    timings = batch.autotune(prisms_per_item=(1024, 4096), blocks_per_cu=(3, 6), use_cache=False)
+    batch.capture()
+    batch.replay()
+    _, dt = _sync_time(batch.replay)
+    log(f"batched: {batch.n_models} models, {batch.pairs:.3e} pairs, {dt:.3f} s")
+    # spot check against single-model launches
+    import harmonica_amd as hm
+
+    res = batch.split()
+    worst = 0.0
+    for b in range(0, batch.n_models, max(batch.n_models // 8, 1)):
+        c, p, d = models[b]
+        ref = hm.prism_gravity(c, p, d, field="g_z")
+        worst = max(worst, float(np.abs(res[b] - ref).max() / np.abs(ref).max()))
+    return {"models": batch.n_models, "pairs": batch.pairs, "s": dt, "pair_evals_per_s": batch.pairs / dt,
+            "prisms_per_item": batch.prisms_per_item, "blocks_per_cu": batch.blocks_per_cu,
+            "autotune_ms": {str(k): v for k, v in timings.items()},
+            "max_rel_diff_vs_single_model": worst}
+
+
+def bench_eqs(args):
+    """Equivalent-source fit of 60,000 scattered g_z data (3 prisms + 1 mGal
+    noise), damping 1e-3, depth 4500 m: FP64 fit vs mixed-precision fit,
+    predictions on a 512 x 512 grid at 1000 m."""
+    import harmonica_amd as hm
+    from harmonica_amd.workloads import eqs_problem
+
+    w = eqs_problem(n_data=args.eqs_data)
+    coords = w["coordinates"]
+    data = hm.prism_gravity(coords, w["prisms"], w["density"], field="g_z") + w["noise"]
+    out = {"n_data": coords[0].size}
+    preds = {}
+    for mode in ("float64", "mixed"):
+        eqs = hm.EquivalentSources(damping=w["damping"], depth=w["depth"], fit_precision=mode)
+        _, dt = _sync_time(lambda: eqs.fit(coords, data))
+        pred, dtp = _sync_time(lambda: eqs.predict(w["grid"]))
+        preds[mode] = pred
+        out[mode] = {"fit_s": dt, "predict_s": dtp, "info": eqs.fit_info_}
+        log(f"eqs {mode}: fit {dt:.2f} s, predict {dtp:.3f} s, {eqs.fit_info_}")
+    ref = preds["float64"]
+    out["mixed_vs_fp64_max_rel_pred_diff"] = float(np.abs(preds["mixed"] - ref).max() / np.abs(ref).max())
+    out["speedup_mixed_vs_fp64"] = out["float64"]["fit_s"] / out["mixed"]["fit_s"]
+    if args.eqs_oracle:
+        import oracle
+
+        n = min(20000, coords[0].size)
+        sub = tuple(c[:n] for c in coords)
+        eqs = hm.EquivalentSources(damping=w["damping"], depth=w["depth"]).fit(sub, data[:n])
+        J = oracle.eqs_jacobian(sub, eqs.points_)
+        t0 = time.perf_counter()
+        coefs = oracle.eqs_fit_coefs(J, data[:n], None, w["damping"])
+        cpu_s = time.perf_counter() - t0
+        del J
+        grid = tuple(c[::4, ::4] for c in w["grid"])
+        p_gpu = eqs.predict(grid)
+        p_cpu = oracle.eqs_predict(grid, eqs.points_, coefs)
+        out["oracle_subset"] = {"n": n, "cpu_fit_s": cpu_s,
+                                "max_rel_pred_diff": float(np.abs(p_gpu - p_cpu).max() / np.abs(p_cpu).max())}
+        log(f"eqs oracle subset: {out['oracle_subset']}")
+    return out
+
+
+def bench_inversion(args):
+    """Differentiable g_z of a 200 x 200 basement layer on a 128 x 128 grid:
+    forward, backward (densities and tops), finite-difference check, 20
+    Gauss-Newton iterations."""
+    import torch
+
+    import harmonica_amd as hm
+    from harmonica_amd import _autograd
+    from harmonica_amd.workloads import inversion_layer
+
+    w = inversion_layer()
+    geom = _autograd.LayerGeometry(w["easting"], w["northing"], w["bottom"], w["coordinates"])
+    dev = lambda a: torch.tensor(a, dtype=torch.float64, device="cuda")
+    rho, top = dev(w["density"]), dev(w["top"])
+    data = geom.forward(rho, top)
+    pairs = geom.n_obs * geom.n_prisms
+    _, t_fwd = _sync_time(lambda: geom.forward(rho, top))
+    v = torch.ones_like(data)
+    _, t_bwd = _sync_time(lambda: geom.vjp(rho, top, v))
+    g_rho, g_top = geom.vjp(rho, top, v)
+    # central differences on three prisms, FP64, steps 1e-4 kg/m^3 and 1e-3 m
+    pts = [c.ravel() for c in w["coordinates"]]
+    he = (w["easting"][1] - w["easting"][0]) / 2
+    errs = []
+    for k, j in ((10, 20), (100, 100), (190, 5)):
+        p = np.array([w["easting"][j] - he, w["easting"][j] + he, w["northing"][k] - he,
+                      w["northing"][k] + he, w["bottom"][k, j], w["top"][k, j]])
+        d = w["density"][k, j]
+        fr = (hm.prism_gravity(pts, p, d + 1e-4, "g_z").sum()
+              - hm.prism_gravity(pts, p, d - 1e-4, "g_z").sum()) / 2e-4
+        pu, pd = p.copy(), p.copy()
+        pu[5] += 1e-3
+        pd[5] -= 1e-3
+        ft = (hm.prism_gravity(pts, pu, d, "g_z").sum() - hm.prism_gravity(pts, pd, d, "g_z").sum()) / 2e-3
+        errs.append(abs(g_rho[k, j].item() - fr) / abs(fr))
+        errs.append(abs(g_top[k, j].item() - ft) / abs(ft))
+    rho0 = torch.full_like(rho, -300.0)
+    top0 = torch.full_like(top, -3500.0)
+    (r, t, hist), t_gn = _sync_time(lambda: _autograd.gauss_newton(geom, data, rho0, top0, n_iterations=20))
+    log(f"inversion: fwd {t_fwd * 1e3:.1f} ms, bwd {t_bwd * 1e3:.1f} ms, GN {t_gn:.2f} s, misfit {hist[0]:.3g} -> {hist[-1]:.3g}")
+    return {"pairs": pairs, "forward_ms": t_fwd * 1e3, "backward_ms": t_bwd * 1e3,
+            "fd_max_rel_err": max(errs), "gauss_newton_s": t_gn, "misfit_mgal": [hist[0], hist[-1]],
+            "iterations": len(hist) - 1}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -68,7 +226,26 @@ def main():
     ap.add_argument("--extra", action="store_true", help="also time the fused ten-field launch")
     ap.add_argument("--obs-side", type=int, default=256)
     ap.add_argument("--prism-side", type=int, default=1000)
+    ap.add_argument("--layer", action="store_true", help="also time the layer accessor path")
+    ap.add_argument("--batched", action="store_true", help="also run the 512-model batch")
+    ap.add_argument("--batched-models", type=int, default=512)
+    ap.add_argument("--eqs", action="store_true", help="also run the equivalent-source fits")
+    ap.add_argument("--eqs-data", type=int, default=60000)
+    ap.add_argument("--eqs-oracle", action="store_true", help="check the FP64 fit on a 20k subset")
+    ap.add_argument("--inversion", action="store_true", help="also run the autograd inversion")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 1 GiB device-to-device (calibrates FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+
+        x = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copied_bytes": 1 << 30}))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path's "
               "multi-GPU sharding is described in DESIGN.md", file=sys.stderr)
@@ -125,13 +302,13 @@ def main():
     value = pairs / (ms * 1e-3)
     log(f"g_z: {ms:.2f} ms/step, {value:.4e} pair-evals/s (wall {wall:.2f} s)")
 
-    fpp = FLOPS_PER_PAIR["g_z"]
-    roofline = None
-    if fpp:
-        achieved = value * fpp
-        roofline = {"bound": "fp64-valu", "achieved": achieved / 1e12, "peak": FP64_PEAK / 1e12,
-                    "unit": "TFLOP/s", "frac": achieved / FP64_PEAK, "traffic": None,
-                    "flops_per_pair": fpp}
+    achieved = value * FLOPS_PER_PAIR_GZ
+    full_size = args.prism_side == 1000 and args.obs_side == 256
+    roofline = {"bound": "fp64-valu", "achieved": achieved / 1e12, "peak": FP64_PEAK / 1e12,
+                "unit": "TFLOP/s", "frac": achieved / FP64_PEAK,
+                "traffic": TRAFFIC_BYTES_GZ if full_size else None,
+                "flops_per_pair": FLOPS_PER_PAIR_GZ,
+                "valu_issue_frac": value * VALU_INSTR_PER_PAIR_GZ / VALU_ISSUE_RATE}
 
     extra = {}
     if args.extra:
@@ -140,6 +317,15 @@ def main():
         ms10, _ = timed(step10, 1, 1)
         extra["all_ten_fields"] = {"ms_per_step": ms10, "pair_evals_per_s": pairs / (ms10 * 1e-3)}
         log(f"ten fields fused: {ms10:.2f} ms/step")
+
+    if args.layer:
+        extra["layer_path"] = bench_layer_fused(args)
+    if args.batched:
+        extra["batched"] = bench_batched(args)
+    if args.eqs:
+        extra["eqs_fit"] = bench_eqs(args)
+    if args.inversion:
+        extra["inversion"] = bench_inversion(args)
 
     cpu = None
     if not args.no_cpu:
