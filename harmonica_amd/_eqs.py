@@ -195,7 +195,7 @@ def fit_device(coords, data, weights, points, damping, precision="float64", max_
 
     mode = _lib.HMX_FIT_FP64 if precision == "float64" else _lib.HMX_FIT_FP32_MFMA
     if max_iter is None:
-        max_iter = 0 if mode == _lib.HMX_FIT_FP64 else 200
+        max_iter = 0 if mode == _lib.HMX_FIT_FP64 else 50
     c = [_dev(x, torch.float64) for x in coords]
     p = [_dev(x, torch.float64) for x in points]
     d = _dev(data, torch.float64)

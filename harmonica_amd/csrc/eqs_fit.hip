@@ -20,6 +20,7 @@
 //   apply_A          FP64 matrix-free A p (J regenerated on the fly) for the
 //                    residuals of the refinement / preconditioned CG.
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 #include "devmath.h"
@@ -144,6 +145,7 @@ __global__ __launch_bounds__(256) void gemm_tn_kernel(const T *__restrict__ A, i
                                                       double beta, double sign, double shift,
                                                       int64_t tiles_n) {
     using acc_t = typename Mfma<T>::acc_t;
+    constexpr bool kFold = !std::is_same<T, double>::value;
     __shared__ T pr[kKC * kPad];
     __shared__ T pc[kKC * kPad];
     int64_t rb, cb;
@@ -163,11 +165,20 @@ __global__ __launch_bounds__(256) void gemm_tn_kernel(const T *__restrict__ A, i
     const int64_t r0 = rb * kST, c0 = cb * kST;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wr = wave >> 1, wc = wave & 1;
+    // FP32 MFMA accumulators are folded into FP64 after every K chunk, so
+    // rounding accumulates over kKC rows only (FP32 inputs, FP64 sums).
     acc_t acc[4][4];
+    double4_t dacc[kFold ? 4 : 1][kFold ? 4 : 1];
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
         for (int c = 0; c < 4; ++c) acc[a][c] = acc_t{0, 0, 0, 0};
+    if constexpr (kFold) {
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) dacc[a][c] = double4_t{0, 0, 0, 0};
+    }
     for (int64_t k0 = 0; k0 < K; k0 += kKC) {
         __syncthreads();
         for (int idx = threadIdx.x; idx < kKC * kST; idx += 256) {
@@ -194,6 +205,16 @@ __global__ __launch_bounds__(256) void gemm_tn_kernel(const T *__restrict__ A, i
                 for (int tj = 0; tj < 4; ++tj)
                     acc[ti][tj] = Mfma<T>::step(a[ti], bb[tj], acc[ti][tj]);
         }
+        if constexpr (kFold) {
+#pragma unroll
+            for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+                for (int tj = 0; tj < 4; ++tj) {
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) dacc[ti][tj][v] += (double)acc[ti][tj][v];
+                    acc[ti][tj] = acc_t{0, 0, 0, 0};
+                }
+        }
     }
 #pragma unroll
     for (int ti = 0; ti < 4; ++ti)
@@ -204,7 +225,9 @@ __global__ __launch_bounds__(256) void gemm_tn_kernel(const T *__restrict__ A, i
                 const int64_t r = r0 + 64 * wr + 16 * ti + Mfma<T>::row(lane, v);
                 const int64_t c = c0 + 64 * wc + 16 * tj + (lane & 15);
                 if (r < M && c < N && (!UPPER || r <= c)) {
-                    double val = sign * (double)acc[ti][tj][v];
+                    const double sum = kFold ? (double)dacc[kFold ? ti : 0][kFold ? tj : 0][v]
+                                             : (double)acc[ti][tj][v];
+                    double val = sign * sum;
                     if (r == c) val += shift;
                     C[r * ldc + c] = beta == 0.0 ? val : fma(beta, C[r * ldc + c], val);
                 }
@@ -795,11 +818,12 @@ int hmx_eqs_fit(const double *easting, const double *northing, const double *upw
     }
     HMX_LAUNCHED("scaled_jacobian_kernel");
 
-    // The FP32 normal matrix carries absolute errors of order
-    // 2^-24 sqrt(n) max(A_jj), which can exceed the damping and break the
-    // factorisation: it is shifted by tau and used as the preconditioner of
-    // CG on the exact FP64 operator.  In either precision a failed
-    // factorisation is retried with a tenfold larger shift.
+    // The normal matrix of FP32 inputs (FP64 sums) differs from the FP64 one
+    // by about 2^-24 of its entries; with damping small against its largest
+    // eigenvalue that can make it indefinite.  A failed factorisation is
+    // retried with a shift tau (then tenfold larger), and the factor of the
+    // (shifted) low-precision matrix serves as the preconditioner of CG on
+    // the exact FP64 operator.
     double tau = 0.0, dmax = 0.0;
     auto diag_max = [&](double *out) -> int {
         if (mixed)
@@ -811,11 +835,6 @@ int hmx_eqs_fit(const double *easting, const double *northing, const double *upw
         HMX_LAUNCHED("colsumsq_kernel");
         return f.reduce(1, wk, nullptr, m, 1, out);
     };
-    if (mixed) {
-        st = diag_max(&dmax);
-        if (st) return st;
-        tau = std::ldexp(4.0 * std::sqrt((double)nd), -24) * dmax;
-    }
     int pivot = 0, attempts = 0;
     for (;;) {
         if (mixed)
@@ -834,7 +853,10 @@ int hmx_eqs_fit(const double *easting, const double *northing, const double *upw
             st = diag_max(&dmax);
             if (st) return st;
         }
-        tau = tau > 0.0 ? 10.0 * tau : 1e-14 * dmax;
+        // First shift: about the rounding of the normal matrix in the mode's
+        // input precision (2^-26 for FP32 inputs with FP64 sums, 1e-14 for
+        // FP64); then tenfold.
+        tau = tau > 0.0 ? 10.0 * tau : (mixed ? std::ldexp(dmax, -26) : 1e-14 * dmax);
     }
 
     // 3. x = (factor)^-1 b.
