@@ -28,20 +28,27 @@ sys.path.insert(0, ROOT)
 # MI355X FP64 vector peak (AMD published: 256 CU x 128 FP64 flop/clk x 2.4 GHz).
 FP64_PEAK = 78.6e12
 
-# FP64 operations per prism-observation pair of the g_z kernel
-# (prism_forward_kernel<g_z>): rocprofv3 counters SQ_INSTS_VALU_{ADD,MUL,
-# TRANS}_F64 + 2 x SQ_INSTS_VALU_FMA_F64, x 64 lanes, divided by the pairs of
-# the profiled launch (profiles/prism_gz_terrain1M_counters.md).  The same
-# profile gives 999.4 VALU instructions per pair; every wave64 VALU
-# instruction occupies a SIMD for 4 cycles, which bounds the pair rate at
-# 1024 SIMDs x 2.4 GHz x 16 / 999.4.
-FLOPS_PER_PAIR_GZ = 1083.7
-VALU_INSTR_PER_PAIR_GZ = 999.4
+# FP64 operations per prism-observation pair executed by the g_z kernel
+# (prism_forward_kernel<g_z>, grouped closed form): rocprofv3 counters
+# SQ_INSTS_VALU_{ADD,MUL,TRANS}_F64 + 2 x SQ_INSTS_VALU_FMA_F64, x 64 lanes,
+# divided by the pairs of the profiled launch
+# (profiles/prism_gz_terrain1M_counters.md).  The same profile gives 478.6
+# VALU instructions per pair; every wave64 VALU instruction occupies a SIMD
+# for 4 cycles, which bounds the pair rate at 1024 SIMDs x 2.4 GHz x 16 /
+# 478.6.  The per-vertex form of the same closed form (8 square roots, 16
+# logarithms, 8 arctangents per pair with this library's routines) measured
+# 1083.7 FP64 operations per pair; it is reported beside the executed count.
+FLOPS_PER_PAIR_GZ = 505.0
+VALU_INSTR_PER_PAIR_GZ = 478.6
+PER_VERTEX_FLOPS_PER_PAIR_GZ = 1083.7
+# Same counters for the ten fields fused (prism_forward_kernel<all>).
+FLOPS_PER_PAIR_ALL = 1106.0
+VALU_INSTR_PER_PAIR_ALL = 1238.8
 VALU_ISSUE_RATE = 1024 * 2.4e9 * 16  # lane-instructions per second, wave64 at 4 cycles
 # HBM bytes per launch of the g_z kernel on this workload: FETCH_SIZE
-# 220218.8 KiB x 2 (a 1 GiB copy reads as 524308 KiB) + WRITE_SIZE 20480 KiB
+# 219591 KiB x 2 (a 1 GiB copy reads as 524308 KiB) + WRITE_SIZE 20480 KiB
 # x 1 (profiles/prism_gz_terrain1M_counters.md).
-TRAFFIC_BYTES_GZ = (220218.8 * 2 + 20480.0) * 1024
+TRAFFIC_BYTES_GZ = (219591.0 * 2 + 20480.0) * 1024
 
 
 def log(msg):
@@ -328,14 +335,19 @@ def main():
                 "unit": "TFLOP/s", "frac": achieved / FP64_PEAK,
                 "traffic": TRAFFIC_BYTES_GZ if full_size else None,
                 "flops_per_pair": FLOPS_PER_PAIR_GZ,
-                "valu_issue_frac": value * VALU_INSTR_PER_PAIR_GZ / VALU_ISSUE_RATE}
+                "valu_issue_frac": value * VALU_INSTR_PER_PAIR_GZ / VALU_ISSUE_RATE,
+                "per_vertex_equivalent_tflops": value * PER_VERTEX_FLOPS_PER_PAIR_GZ / 1e12}
 
     extra = {}
     if args.extra:
         mask = (1 << 10) - 1
         step10, _ = make_step(mask, 10)
         ms10, _ = timed(step10, 1, 1)
-        extra["all_ten_fields"] = {"ms_per_step": ms10, "pair_evals_per_s": pairs / (ms10 * 1e-3)}
+        rate10 = pairs / (ms10 * 1e-3)
+        extra["all_ten_fields"] = {"ms_per_step": ms10, "pair_evals_per_s": rate10,
+                                   "fp64_tflops": rate10 * FLOPS_PER_PAIR_ALL / 1e12,
+                                   "fp64_peak_frac": rate10 * FLOPS_PER_PAIR_ALL / FP64_PEAK,
+                                   "valu_issue_frac": rate10 * VALU_INSTR_PER_PAIR_ALL / VALU_ISSUE_RATE}
         log(f"ten fields fused: {ms10:.2f} ms/step")
 
     if args.layer:

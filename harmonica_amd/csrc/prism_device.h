@@ -10,13 +10,23 @@
 //   L_x = ln(x + r), evaluated as ln((y^2 + z^2) / (r - x)) for x < 0 to avoid
 //         the cancellation of x + r, and 0 where its argument is 0;
 //   A_x = atan(y z / (x r)) as a principal value, +-pi/2 or 0 where x r = 0;
-// and L_y, L_z, A_y, A_z by cyclic permutation.  Field slots follow the
-// reference's FIELDS table (gravity.py:37-48):
+// and L_y, L_z, A_y, A_z by cyclic permutation (below, "safe_log" and
+// "safe_atan" name these conventions).  Field slots follow the reference's
+// FIELDS table (gravity.py:37-48):
 //   potential  x y L_z + y z L_x + z x L_y - (x^2 A_x + y^2 A_y + z^2 A_z) / 2
 //   g_e  -(y L_z + z L_y - x A_x)    g_n  -(z L_x + x L_z - y A_y)
 //   g_u  -(x L_y + y L_x - z A_z)    g_ee -A_x   g_nn -A_y   g_uu -A_z
 //   g_en  L_z    g_eu  L_y    g_nu  L_x
 // Tensor terms are NaN on their singular points (gravity.py:141-158).
+//
+// The kernels here do not evaluate the 8 vertices one by one: the terms are
+// grouped by the coordinates their coefficients depend on, so that each
+// group of logarithms becomes the logarithm of one quotient and each group
+// of arctangents one two-argument arctangent (the signed solid angle of a
+// face).  The sums are the same; the grouped quantities are formed without
+// the cancellation between vertex terms of distant prisms, and they need
+// 12 logarithms and 6 arctangents for all ten fields (4 and 2 for g_z)
+// instead of 24 and 24 (16 and 8).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -48,66 +58,141 @@ struct FieldSet {
     static constexpr int slot(int f) { return __builtin_popcount(M & ((1u << f) - 1u)); }
 };
 
-// Principal value of atan(a / b); b == 0 gives +-pi/2 by the sign of a, or 0.
-__device__ __forceinline__ double safe_atan(double a, double b, const dm::Tables &tab) {
-    return dm::atan_ratio(a, b, tab);
-}
-
-// ln(x + r); the x < 0 branch uses (y^2 + z^2) / (r - x); 0 where the
-// argument is 0.
-__device__ __forceinline__ double safe_log(double x, double y, double z, double r,
+// sum_k (-1)^k ln(P(0,k) / P(1,k)) for four safe_log arguments
+// P(j,k) = a_j + r_jk if a_j >= 0, else w2_k / (r_jk - a_j), where a is the
+// logarithm's own coordinate (a_0 >= a_1) and w2_k the sum of squares of the
+// other two.  With t_jk = r_jk + |a_j| (no cancellation), P = t if a >= 0
+// and w2 / t if a < 0, so the four logarithms reduce to one:
+//   a_1 >= 0:         ln(t00 t11 / (t10 t01))
+//   a_0 < 0:         -ln(t00 t11 / (t10 t01))
+//   a_0 >= 0 > a_1:   ln(t00 t10 w2_1 / (t01 t11 w2_0)).
+// An argument can vanish only where the logarithm's coefficient is zero;
+// the caller drops those terms.
+__device__ __forceinline__ double log_quad(double a0, double a1, double r00, double r01,
+                                           double r10, double r11, double w20, double w21,
                                            const dm::Tables &tab) {
-    double arg;
-    if (x < 0.0) {
-        arg = dm::div(fma(y, y, z * z), r - x);
-    } else {
-        arg = x + r;
-    }
-    const double l = dm::log_pos(arg, tab);
-    return arg == 0.0 ? 0.0 : l;
+    const bool pos1 = a1 >= 0.0, neg0 = a0 < 0.0, mixed = !pos1 && !neg0;
+    const double t00 = r00 + fabs(a0), t01 = r01 + fabs(a0);
+    const double t10 = r10 + fabs(a1), t11 = r11 + fabs(a1);
+    const double num = mixed ? t00 * t10 * w21 : t00 * t11;
+    const double den = mixed ? t01 * t11 * w20 : t10 * t01;
+    const double l = dm::log_pos(num * dm::rcp(den), tab);
+    return neg0 ? -l : l;
 }
 
-// Accumulate into vs[] the kernel values of one vertex times its sign sg
-// (+1 or -1).
-template <unsigned M>
-__device__ __forceinline__ void vertex(double x, double y, double z, double x2, double y2,
-                                       double z2, double sg, double (&vs)[FieldSet<M>::count],
-                                       const dm::Tables &tab) {
-    using F = FieldSet<M>;
-    const double r = dm::sqrt_nn(x2 + y2 + z2);
-    double Lx = 0.0, Ly = 0.0, Lz = 0.0, Ax = 0.0, Ay = 0.0, Az = 0.0;
-    if constexpr (F::Lx) Lx = safe_log(x, y, z, r, tab);
-    if constexpr (F::Ly) Ly = safe_log(y, z, x, r, tab);
-    if constexpr (F::Lz) Lz = safe_log(z, x, y, r, tab);
-    if constexpr (F::Ax) Ax = safe_atan(y * z, x * r, tab);
-    if constexpr (F::Ay) Ay = safe_atan(z * x, y * r, tab);
-    if constexpr (F::Az) Az = safe_atan(x * y, z * r, tab);
-    auto add = [&](int f, double k) { vs[F::slot(f)] = fma(sg, k, vs[F::slot(f)]); };
-    if constexpr (F::pot) {
-        double k = 0.0;
-        k += (x != 0.0 && y != 0.0) ? x * y * Lz : 0.0;
-        k += (y != 0.0 && z != 0.0) ? y * z * Lx : 0.0;
-        k += (z != 0.0 && x != 0.0) ? z * x * Ly : 0.0;
-        k -= (x != 0.0) ? 0.5 * x2 * Ax : 0.0;
-        k -= (y != 0.0) ? 0.5 * y2 * Ay : 0.0;
-        k -= (z != 0.0) ? 0.5 * z2 * Az : 0.0;
-        add(0, k);
-    }
-    if constexpr (F::ge) add(1, -(y * Lz + z * Ly - x * Ax));
-    if constexpr (F::gn) add(2, -(z * Lx + x * Lz - y * Ay));
-    if constexpr (F::gu) add(3, -(x * Ly + y * Lx - z * Az));
-    if constexpr (F::ee) add(4, -Ax);
-    if constexpr (F::nn) add(5, -Ay);
-    if constexpr (F::uu) add(6, -Az);
-    if constexpr (F::en) add(7, Lz);
-    if constexpr (F::eu) add(8, Ly);
-    if constexpr (F::nu) add(9, Lx);
+// sum_{i,j} (-1)^(i+j) A_z(i, j) on the face at height z != 0 (r_ij the
+// vertex distances): the signed solid angle of the face rectangle.  With
+// a_i = x_i y / (z r_i) for one y, tan(A_0 - A_1) = (a_0 - a_1) / (1 + a_0 a_1)
+// and 1 + a_0 a_1 has the sign of cos(A_0 - A_1) (cos A_i > 0), so each
+// difference over i is one two-argument angle (N_j, D_j) (both scaled by
+// z^2 r_0 r_1 > 0), and the difference of the two over j is again one angle
+// (N_0 D_1 - N_1 D_0, D_0 D_1 + N_0 N_1), correct modulo 2 pi.  The solid
+// angle lies in (-pi, pi) unless the point projects strictly inside the
+// rectangle, where it has the sign of z and magnitude up to 2 pi; that case
+// takes the branch with the sign of z.
+__device__ __forceinline__ double face_angle(double x0, double x1, double y0, double y1, double z,
+                                             double r00, double r01, double r10, double r11,
+                                             bool inside, const dm::Tables &tab) {
+    const double n0 = y0 * z * fma(x0, r10, -x1 * r00);
+    const double d0 = fma(z * z, r00 * r10, x0 * x1 * (y0 * y0));
+    const double n1 = y1 * z * fma(x0, r11, -x1 * r01);
+    const double d1 = fma(z * z, r01 * r11, x0 * x1 * (y1 * y1));
+    const double a = dm::atan2_full(fma(n0, d1, -n1 * d0), fma(d0, d1, n0 * n1), tab);
+    constexpr double kTwoPi = 6.283185307179586232;
+    const bool wrap = inside && (z > 0.0 ? a < 0.0 : a > 0.0);
+    return wrap ? (z > 0.0 ? a + kTwoPi : a - kTwoPi) : a;
 }
 
-// Vertices evaluated per loop iteration: 4 for one field (register use
-// allows 3 waves per SIMD), 2 for several fields (more live accumulators).
-template <unsigned M>
-constexpr int kVertexUnroll = FieldSet<M>::count == 1 ? 4 : 2;
+// ln(P_0 / P_1) for two safe_log arguments that differ only in the
+// logarithm's own coordinate a (a_0 >= a_1), P = a + r (a >= 0) or
+// w2 / (r - a) (a < 0), w2 the sum of squares of the other two coordinates.
+// With t = r + |a|: P = t for a >= 0 and w2 / t for a < 0, so
+//   a_1 >= 0: ln(t_0 / t_1);  a_0 < 0: -ln(t_0 / t_1);  else ln(t_0 t_1 / w2).
+// On the line of an edge (w2 = 0) safe_log's arguments are 2a (a > 0) or 0
+// (a <= 0, logarithm 0): ln(t_0 / t_1) if a_1 > 0, ln(t_0) if only a_0 > 0,
+// else 0.  The result is finite for every input.
+__device__ __forceinline__ double log_diff(double a0, double a1, double r0, double r1, double w2,
+                                           const dm::Tables &tab) {
+    const double t0 = r0 + fabs(a0), t1 = r1 + fabs(a1);
+    const bool line = w2 == 0.0;
+    const bool mixed = a0 >= 0.0 && a1 < 0.0;
+    const double num = line ? (a0 > 0.0 ? t0 : 1.0) : (mixed ? t0 * t1 : t0);
+    const double den = line ? (a1 > 0.0 ? t1 : 1.0) : (mixed ? w2 : t1);
+    const double l = dm::log_pos(num * dm::rcp(den), tab);
+    return (a0 < 0.0 && !line) ? -l : l;
+}
+
+// Signed solid angle sum_{i,j} (-1)^(i+j) atan(a_i b_j / (c r_ij)) of the
+// face at height c, including c = 0 where safe_atan gives +-pi/2 by sign.
+__device__ __forceinline__ double face_sum(double a0, double a1, double b0, double b1, double c,
+                                           double r00, double r01, double r10, double r11,
+                                           const dm::Tables &tab) {
+    const bool inside = a0 > 0.0 && a1 < 0.0 && b0 > 0.0 && b1 < 0.0;
+    const double w = face_angle(a0, a1, b0, b1, c, r00, r01, r10, r11, inside, tab);
+    const double sa = (double)((a0 > 0.0) - (a0 < 0.0) - (a1 > 0.0) + (a1 < 0.0));
+    const double sb = (double)((b0 > 0.0) - (b0 < 0.0) - (b1 > 0.0) + (b1 < 0.0));
+    return c == 0.0 ? 1.5707963267948966 * sa * sb : w;
+}
+
+// Sum over the 8 vertices of (-1)^(i+j+k) K_u, K_u = -(x L_y + y L_x - z A_z),
+// with the vertices grouped:
+//   sum x L_y = sum_i (-1)^i x_i sum_k (-1)^k ln(P(i,0,k) / P(i,1,k))
+//   sum y L_x = sum_j (-1)^j y_j sum_k (-1)^k ln(P(0,j,k) / P(1,j,k))
+//   sum z A_z = sum_k (-1)^k z_k sum_{i,j} (-1)^(i+j) A_z(i,j,k),
+// i.e. 4 logarithms and 2 arctangents per prism instead of 16 and 8.  The
+// grouped terms are formed without the cancellation between vertex terms
+// of far prisms, so they are also more accurate.
+// Terms whose coefficient (x_i, y_j or z_k) is zero are zero, as in the
+// per-vertex form.  With kTop, *ftop receives the top-face sum
+// sum_{i,j} (-1)^(i+j) A_z(i, j, 0) (the derivative with respect to the top).
+template <bool kTop>
+__device__ __forceinline__ double gu_sum_paired(double x0, double x1, double y0, double y1,
+                                                double z0, double z1, const dm::Tables &tab,
+                                                double *ftop) {
+    const double xs0 = x0 * x0, xs1 = x1 * x1, ys0 = y0 * y0, ys1 = y1 * y1;
+    const double zs0 = z0 * z0, zs1 = z1 * z1;
+    double r[2][2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int k = 0; k < 2; ++k)
+                r[i][j][k] = dm::sqrt_nn((i ? xs1 : xs0) + (j ? ys1 : ys0) + (k ? zs1 : zs0));
+    double sum = 0.0, top = 0.0;
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const double c = i ? x1 : x0, cs = i ? xs1 : xs0;
+        const double l = log_quad(y0, y1, r[i][0][0], r[i][0][1], r[i][1][0], r[i][1][1],
+                                  cs + zs0, cs + zs1, tab);
+        sum = c == 0.0 ? sum : fma(i ? -c : c, l, sum);
+    }
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        const double c = j ? y1 : y0, cs = j ? ys1 : ys0;
+        const double l = log_quad(x0, x1, r[0][j][0], r[0][j][1], r[1][j][0], r[1][j][1],
+                                  cs + zs0, cs + zs1, tab);
+        sum = c == 0.0 ? sum : fma(j ? -c : c, l, sum);
+    }
+    const bool inside = x0 > 0.0 && x1 < 0.0 && y0 > 0.0 && y1 < 0.0;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const double z = k ? z1 : z0;
+        const double w = face_angle(x0, x1, y0, y1, z, r[0][0][k], r[0][1][k], r[1][0][k],
+                                    r[1][1][k], inside, tab);
+        sum = z == 0.0 ? sum : fma(k ? z : -z, w, sum);
+        if constexpr (kTop) {
+            if (k == 0) {
+                // z_T = 0: safe_atan(x y, 0) = +-pi/2 by the sign of x y, or 0.
+                const double sx = (x0 > 0.0) - (x0 < 0.0) - ((x1 > 0.0) - (x1 < 0.0));
+                const double sy = (y0 > 0.0) - (y0 < 0.0) - ((y1 > 0.0) - (y1 < 0.0));
+                top = z == 0.0 ? 1.5707963267948966 * sx * sy : w;
+            }
+        }
+    }
+    if constexpr (kTop) *ftop = top;
+    return -sum;
+}
 
 // Add the fields of one prism (bounds p[0..5] = W, E, S, N, B, T; grho =
 // G * density) at (e, n, u) to acc[]; OR singular-point bits into *sflags.
@@ -117,26 +202,88 @@ __device__ __forceinline__ void prism_pair(double e, double n, double u, double 
                                            double (&acc)[FieldSet<M>::count], uint32_t &sflags,
                                            const dm::Tables &tab) {
     using F = FieldSet<M>;
+    if constexpr (M == 0x8u) {  // g_u alone
+        acc[0] += grho * gu_sum_paired<false>(E - e, W - e, N - n, S - n, T - u, B - u, tab,
+                                              nullptr);
+        return;
+    }
+    if constexpr (M == 0x2u) {  // g_e alone: K_e is K_u with (x, y, z) -> (y, z, x)
+        acc[0] += grho * gu_sum_paired<false>(N - n, S - n, T - u, B - u, E - e, W - e, tab,
+                                              nullptr);
+        return;
+    }
+    if constexpr (M == 0x4u) {  // g_n alone: K_n is K_u with (x, y, z) -> (z, x, y)
+        acc[0] += grho * gu_sum_paired<false>(T - u, B - u, E - e, W - e, N - n, S - n, tab,
+                                              nullptr);
+        return;
+    }
     double vs[F::count];
-#pragma unroll
-    for (int s = 0; s < F::count; ++s) vs[s] = 0.0;
     const double x0 = E - e, x1 = W - e;
     const double y0 = N - n, y1 = S - n;
     const double z0 = T - u, z1 = B - u;
-    const double x0s = x0 * x0, x1s = x1 * x1, y0s = y0 * y0, y1s = y1 * y1;
-    const double z0s = z0 * z0, z1s = z1 * z1;
-    // Vertex order of the reference's loop: east, north, top first; sign
-    // (-1)^(i + j + k).  A partially unrolled loop bounds register pressure
-    // (occupancy) while leaving the scheduler independent vertices.
-#pragma unroll kVertexUnroll<M>
-    for (int v = 0; v < 8; ++v) {
-        const bool bi = v & 4, bj = v & 2, bk = v & 1;
-        const double x = bi ? x1 : x0, xs = bi ? x1s : x0s;
-        const double y = bj ? y1 : y0, ys = bj ? y1s : y0s;
-        const double z = bk ? z1 : z0, zs = bk ? z1s : z0s;
-        const double sg = (__builtin_popcount(v) & 1) ? -1.0 : 1.0;
-        vertex<M>(x, y, z, xs, ys, zs, sg, vs, tab);
+    const double xs[2] = {x0 * x0, x1 * x1}, ys[2] = {y0 * y0, y1 * y1};
+    const double zs[2] = {z0 * z0, z1 * z1};
+    const double X[2] = {x0, x1}, Y[2] = {y0, y1}, Z[2] = {z0, z1};
+    double r[2][2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int k = 0; k < 2; ++k) r[i][j][k] = dm::sqrt_nn(xs[i] + ys[j] + zs[k]);
+    // Vertex sums grouped by the coordinates each term depends on:
+    //   sum (-1)^(i+j+k) f(y_j, z_k) L_x = sum_{j,k} (-1)^(j+k) f(y_j, z_k) DLx[j][k]
+    //   with DLx[j][k] = L_x(0,j,k) - L_x(1,j,k) (one logarithm), and likewise
+    //   DLy[i][k], DLz[i][j];
+    //   sum (-1)^(i+j+k) f(x_i) A_x = sum_i (-1)^i f(x_i) Ox[i], Ox[i] the
+    //   face solid angle (one arctangent), and likewise Oy[j], Oz[k].
+    double DLx[2][2] = {}, DLy[2][2] = {}, DLz[2][2] = {}, Ox[2] = {}, Oy[2] = {}, Oz[2] = {};
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            if constexpr (F::Lx)
+                DLx[a][b] = log_diff(x0, x1, r[0][a][b], r[1][a][b], ys[a] + zs[b], tab);
+            if constexpr (F::Ly)
+                DLy[a][b] = log_diff(y0, y1, r[a][0][b], r[a][1][b], xs[a] + zs[b], tab);
+            if constexpr (F::Lz)
+                DLz[a][b] = log_diff(z0, z1, r[a][b][0], r[a][b][1], xs[a] + ys[b], tab);
+        }
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+        if constexpr (F::Ax)
+            Ox[a] = face_sum(y0, y1, z0, z1, X[a], r[a][0][0], r[a][0][1], r[a][1][0], r[a][1][1], tab);
+        if constexpr (F::Ay)
+            Oy[a] = face_sum(z0, z1, x0, x1, Y[a], r[0][a][0], r[1][a][0], r[0][a][1], r[1][a][1], tab);
+        if constexpr (F::Az)
+            Oz[a] = face_sum(x0, x1, y0, y1, Z[a], r[0][0][a], r[0][1][a], r[1][0][a], r[1][1][a], tab);
     }
+    // Signed sums with one coefficient (or none) per grouped quantity.
+    auto s2 = [](const double (&d)[2][2], double c0, double c1, bool by_first) {
+        // sum_{a,b} (-1)^(a+b) c_(a or b) d[a][b]
+        return by_first ? c0 * (d[0][0] - d[0][1]) - c1 * (d[1][0] - d[1][1])
+                        : c0 * (d[0][0] - d[1][0]) - c1 * (d[0][1] - d[1][1]);
+    };
+    auto s1 = [](const double (&o)[2], double c0, double c1) { return c0 * o[0] - c1 * o[1]; };
+    if constexpr (F::pot) {
+        double k = x0 * (y0 * DLz[0][0] - y1 * DLz[0][1]) - x1 * (y0 * DLz[1][0] - y1 * DLz[1][1]);
+        k += y0 * (z0 * DLx[0][0] - z1 * DLx[0][1]) - y1 * (z0 * DLx[1][0] - z1 * DLx[1][1]);
+        k += x0 * (z0 * DLy[0][0] - z1 * DLy[0][1]) - x1 * (z0 * DLy[1][0] - z1 * DLy[1][1]);
+        k -= 0.5 * (s1(Ox, xs[0], xs[1]) + s1(Oy, ys[0], ys[1]) + s1(Oz, zs[0], zs[1]));
+        vs[F::slot(0)] = k;
+    }
+    if constexpr (F::ge)
+        vs[F::slot(1)] = -(s2(DLz, y0, y1, false) + s2(DLy, z0, z1, false) - s1(Ox, x0, x1));
+    if constexpr (F::gn)
+        vs[F::slot(2)] = -(s2(DLx, z0, z1, false) + s2(DLz, x0, x1, true) - s1(Oy, y0, y1));
+    if constexpr (F::gu)
+        vs[F::slot(3)] = -(s2(DLy, x0, x1, true) + s2(DLx, y0, y1, true) - s1(Oz, z0, z1));
+    if constexpr (F::ee) vs[F::slot(4)] = -(Ox[0] - Ox[1]);
+    if constexpr (F::nn) vs[F::slot(5)] = -(Oy[0] - Oy[1]);
+    if constexpr (F::uu) vs[F::slot(6)] = -(Oz[0] - Oz[1]);
+    if constexpr (F::en) vs[F::slot(7)] = s2(DLz, 1.0, 1.0, true);
+    if constexpr (F::eu) vs[F::slot(8)] = s2(DLy, 1.0, 1.0, true);
+    if constexpr (F::nu) vs[F::slot(9)] = s2(DLx, 1.0, 1.0, true);
     if constexpr (F::tensor) {
         const bool in_e = W <= e && e <= E, in_n = S <= n && n <= N, in_u = B <= u && u <= T;
         const bool at_e = e == W || e == E, at_n = n == S || n == N, at_u = u == B || u == T;

@@ -28,26 +28,7 @@ namespace {
 __device__ __forceinline__ void gz_sums(double e, double n, double u, double W, double E, double S,
                                         double N, double B, double T, const dm::Tables &tab,
                                         double &s_out, double &f_out) {
-    const double x0 = E - e, x1 = W - e, y0 = N - n, y1 = S - n, z0 = T - u, z1 = B - u;
-    const double x0s = x0 * x0, x1s = x1 * x1, y0s = y0 * y0, y1s = y1 * y1;
-    const double z0s = z0 * z0, z1s = z1 * z1;
-    double sum = 0.0, ftop = 0.0;
-#pragma unroll 2
-    for (int v = 0; v < 8; ++v) {
-        const bool bi = v & 4, bj = v & 2, bk = v & 1;
-        const double x = bi ? x1 : x0, xs = bi ? x1s : x0s;
-        const double y = bj ? y1 : y0, ys = bj ? y1s : y0s;
-        const double z = bk ? z1 : z0, zs = bk ? z1s : z0s;
-        const double sg = (__builtin_popcount(v) & 1) ? -1.0 : 1.0;
-        const double r = dm::sqrt_nn(xs + ys + zs);
-        const double Lx = dev::safe_log(x, y, z, r, tab);
-        const double Ly = dev::safe_log(y, z, x, r, tab);
-        const double Az = dev::safe_atan(x * y, z * r, tab);
-        sum = fma(sg, -(x * Ly + y * Lx - z * Az), sum);
-        if (!bk) ftop = fma(sg, Az, ftop);
-    }
-    s_out = sum;
-    f_out = ftop;
+    s_out = dev::gu_sum_paired<true>(E - e, W - e, N - n, S - n, T - u, B - u, tab, &f_out);
 }
 
 struct alignas(16) StagedGradPrism {

@@ -20,7 +20,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 mpmath.mp.dps = 50
 
-SQRT, DIV, LOG, ATAN = 0, 1, 2, 3
+SQRT, DIV, LOG, ATAN, ATAN2 = 0, 1, 2, 3, 4
 
 
 @pytest.fixture(scope="module")
@@ -113,3 +113,23 @@ def test_atan_ratio(ev):
                  np.array([0.0, 0.0, 0.0, 4.0, -0.0, -0.0]))
     half_pi = np.pi / 2
     np.testing.assert_array_equal(special, [half_pi, -half_pi, 0.0, 0.0, half_pi, -half_pi])
+
+
+def test_atan2(ev):
+    """Two-argument arctangent in (-pi, pi]: every octant, both signs of
+    zero in the first argument, exact axes."""
+    rng = np.random.default_rng(4)
+    a = _wide(rng, 3000, 1e-9, 1e9) * rng.choice([-1, 1], 3000)
+    b = _wide(rng, 3000, 1e-9, 1e9) * rng.choice([-1, 1], 3000)
+    got = ev(ATAN2, a, b)
+    worst = 0.0
+    for g, x, y in zip(got, a, b):
+        exact = mpmath.atan2(mpmath.mpf(x), mpmath.mpf(y))
+        err = abs(mpmath.mpf(float(g)) - exact)
+        bound = 1.5 * np.spacing(abs(float(exact))) + 2.0**-53
+        worst = max(worst, float(err / bound))
+    assert worst <= 1.0, worst
+    special = ev(ATAN2, np.array([0.0, 0.0, 3.0, -3.0, 2.0, -2.0]),
+                 np.array([1.0, -1.0, 0.0, 0.0, -2.0, -2.0]))
+    np.testing.assert_allclose(special, [0.0, np.pi, np.pi / 2, -np.pi / 2, 0.75 * np.pi,
+                                         -0.75 * np.pi], rtol=1e-15, atol=0)

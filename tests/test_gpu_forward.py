@@ -71,6 +71,28 @@ def test_prism_gravity_vs_oracle(hm, field, seed):
     assert_parity(got, oracle.prism_gravity(coords, prisms, dens, field), err_msg=field)
 
 
+@pytest.mark.parametrize("field", ["g_z", "g_zz", "potential"])
+def test_points_on_prism_planes(hm, field):
+    """Observation points snapped onto the planes of the prism faces (on
+    faces, edges, vertices, their extensions and inside): the grouped g_z
+    form and the per-vertex forms against the oracle."""
+    rng = np.random.default_rng(21)
+    prism = np.array([-40.0, 60.0, -30.0, 50.0, -80.0, 20.0])
+    n = 4000
+    pts = rng.uniform(-150, 150, (3, n))
+    for ax in range(3):
+        lo, hi = prism[2 * ax], prism[2 * ax + 1]
+        u = rng.uniform(size=n)
+        pts[ax] = np.where(u < 0.2, lo, np.where(u < 0.4, hi, pts[ax]))
+    coords = tuple(pts)
+    if field == "g_zz":
+        # tensor components are NaN on edges and vertices; checks disabled
+        got = hm.prism_gravity(coords, prism, 2670.0, field=field, disable_checks=True)
+    else:
+        got = hm.prism_gravity(coords, prism, 2670.0, field=field)
+    assert_parity(got, oracle.prism_gravity(coords, prism, [2670.0], field), err_msg=field)
+
+
 def test_singular_points_nan_and_warning(hm):
     """test_prism.py:380-502 geometry: vertices and edge centres."""
     p = np.array([-10.3, 5.4, 8.6, 14.3, -30.3, 2.4])
