@@ -33,6 +33,7 @@ typedef double double4_t __attribute__((ext_vector_type(4)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
 
 constexpr int kNB = 128;    // Cholesky block size
+constexpr int kOuter = 1024;  // Cholesky outer block (K of the big trailing updates)
 constexpr int kST = 128;    // SYRK block tile (kST x kST per workgroup of 256 threads)
 constexpr int kKC = 16;     // SYRK K chunk staged in LDS
 constexpr int kPad = 144;   // LDS row stride (doubles/floats) of the SYRK panels
@@ -129,67 +130,137 @@ struct Mfma<float> {
     static __device__ __forceinline__ int row(int lane, int v) { return 4 * (lane >> 4) + v; }
 };
 
-// C = beta * C + sign * A^T B + shift * I on the selected tiles, with A
-// (K x M) and B (K x N) row-major (leading dimensions lda, ldb) and C (M x N)
-// row-major (ldc).  UPPER: M == N and only the upper triangle (r <= c) is
-// computed (the symmetric rank-K update A^T A when B == A).  A workgroup of
-// 4 wavefronts owns a kST x kST tile of C (2 x 2 waves of 64 x 64, each
-// 4 x 4 MFMA 16x16x4 tiles); the K dimension is staged through LDS in
-// chunks of kKC rows.  Every workgroup reads all its inputs before it writes
-// its tile, and tiles are disjoint, so C may alias B when M <= kST (used by
-// the in-place panel solve of the Cholesky factorisation).
-template <class T, bool UPPER>
-__global__ __launch_bounds__(256) void gemm_tn_kernel(const T *__restrict__ A, int64_t lda,
-                                                      const T *B, int64_t ldb, int64_t K,
-                                                      int64_t M, int64_t N, double *C, int64_t ldc,
-                                                      double beta, double sign, double shift,
-                                                      int64_t tiles_n) {
+// Rows [k0, k0 + kKC) x columns [col0, col0 + kST) of a row-major K x cols
+// panel, fetched 8 consecutive columns per thread (16-byte loads when the
+// panel is aligned, zero outside the panel).
+template <class T>
+struct Panel {
+    const T *X;
+    int64_t ld, rows, cols;
+    bool vec;  // X and ld * sizeof(T) are multiples of 16 bytes
+    // Unchecked 16-byte loads: the caller guarantees the whole kKC x kST
+    // block is inside the panel and vec is set.
+    __device__ __forceinline__ void fetch_fast(int64_t k0, int64_t col0, int kk, int c8,
+                                               T (&v)[8]) const {
+        const T *p = X + (k0 + kk) * ld + col0 + 8 * c8;
+        if constexpr (sizeof(T) == 8) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const double2 d = reinterpret_cast<const double2 *>(p)[i];
+                v[2 * i] = d.x;
+                v[2 * i + 1] = d.y;
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                const float4 f = reinterpret_cast<const float4 *>(p)[i];
+                v[4 * i] = f.x;
+                v[4 * i + 1] = f.y;
+                v[4 * i + 2] = f.z;
+                v[4 * i + 3] = f.w;
+            }
+        }
+    }
+    __device__ __forceinline__ void fetch(int64_t k0, int64_t col0, int kk, int c8,
+                                          T (&v)[8]) const {
+        const int64_t k = k0 + kk, c = col0 + 8 * c8;
+        if (k < rows && c + 8 <= cols && vec) {
+            const T *p = X + k * ld + c;
+            if constexpr (sizeof(T) == 8) {
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const double2 d = reinterpret_cast<const double2 *>(p)[i];
+                    v[2 * i] = d.x;
+                    v[2 * i + 1] = d.y;
+                }
+            } else {
+#pragma unroll
+                for (int i = 0; i < 2; ++i) {
+                    const float4 f = reinterpret_cast<const float4 *>(p)[i];
+                    v[4 * i] = f.x;
+                    v[4 * i + 1] = f.y;
+                    v[4 * i + 2] = f.z;
+                    v[4 * i + 3] = f.w;
+                }
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) v[i] = (k < rows && c + i < cols) ? X[k * ld + c + i] : T(0);
+        }
+    }
+};
+
+template <class T>
+Panel<T> make_panel(const T *X, int64_t ld, int64_t rows, int64_t cols) {
+    const bool vec = ((uintptr_t)X % 16 == 0) && (((size_t)ld * sizeof(T)) % 16 == 0);
+    return Panel<T>{X, ld, rows, cols, vec};
+}
+
+// The K loop of one kST x kST tile (see gemm_tn_kernel).  FAST: every chunk
+// is inside both panels and they are aligned, so the loads are unchecked
+// 16-byte loads; the loop body is straight-line code either way, which
+// keeps the accumulators in place across iterations.
+template <class T, bool FAST, bool UPPER>
+__device__ __forceinline__ void tile_k_loop(const Panel<T> &A, const Panel<T> &B, int64_t K,
+                                            int64_t r0, int64_t c0, T (*lds)[2][kKC * kPad],
+                                            int64_t M, int64_t N, double *C, int64_t ldc,
+                                            double beta, double sign, double shift) {
     using acc_t = typename Mfma<T>::acc_t;
     constexpr bool kFold = !std::is_same<T, double>::value;
-    __shared__ T pr[kKC * kPad];
-    __shared__ T pc[kKC * kPad];
-    int64_t rb, cb;
-    if (UPPER) {
-        int64_t b = blockIdx.x, rem = tiles_n;
-        rb = 0;
-        while (b >= rem) {
-            b -= rem;
-            ++rb;
-            --rem;
-        }
-        cb = rb + b;
-    } else {
-        rb = blockIdx.x / tiles_n;
-        cb = blockIdx.x - rb * tiles_n;
-    }
-    const int64_t r0 = rb * kST, c0 = cb * kST;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int wr = wave >> 1, wc = wave & 1;
-    // FP32 MFMA accumulators are folded into FP64 after every K chunk, so
-    // rounding accumulates over kKC rows only (FP32 inputs, FP64 sums).
     acc_t acc[4][4];
     double4_t dacc[kFold ? 4 : 1][kFold ? 4 : 1];
+    // FP64 accumulators are carried across the whole K loop; their zeros
+    // come out of the matrix unit itself so they live in the MFMA result
+    // registers from the start (a plain zero would be a VGPR value and cost
+    // a copy per iteration).  FP32 accumulators restart every chunk.
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
-        for (int c = 0; c < 4; ++c) acc[a][c] = acc_t{0, 0, 0, 0};
+        for (int c = 0; c < 4; ++c)
+            acc[a][c] = kFold ? acc_t{0, 0, 0, 0}
+                              : Mfma<T>::step(T(0), T(0), acc_t{0, 0, 0, 0});
     if constexpr (kFold) {
 #pragma unroll
         for (int a = 0; a < 4; ++a)
 #pragma unroll
             for (int c = 0; c < 4; ++c) dacc[a][c] = double4_t{0, 0, 0, 0};
     }
-    for (int64_t k0 = 0; k0 < K; k0 += kKC) {
-        __syncthreads();
-        for (int idx = threadIdx.x; idx < kKC * kST; idx += 256) {
-            const int kk = idx / kST, cc = idx - kk * kST;
-            const int64_t k = k0 + kk;
-            const bool kin = k < K;
-            const int64_t gr = r0 + cc, gc = c0 + cc;
-            pr[kk * kPad + cc] = (kin && gr < M) ? A[k * lda + gr] : T(0);
-            pc[kk * kPad + cc] = (kin && gc < N) ? B[k * ldb + gc] : T(0);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wr = wave >> 1, wc = wave & 1;
+    const int kk = threadIdx.x >> 4, c8 = threadIdx.x & 15;  // staging slot
+    T ra[8], rc[8];
+    auto fetch = [&](int64_t k0) {
+        if constexpr (FAST) {
+            A.fetch_fast(k0, r0, kk, c8, ra);
+            B.fetch_fast(k0, c0, kk, c8, rc);
+        } else {
+            A.fetch(k0, r0, kk, c8, ra);
+            B.fetch(k0, c0, kk, c8, rc);
         }
-        __syncthreads();
+    };
+    auto store = [&](int buf) {
+        T *pa = &lds[buf][0][kk * kPad + 8 * c8];
+        T *pb = &lds[buf][1][kk * kPad + 8 * c8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            pa[i] = ra[i];
+            pb[i] = rc[i];
+        }
+    };
+    // Chunk count as a wave-uniform 32-bit value (K < 2^31 * kKC), so the
+    // loop control stays scalar.
+    const int chunks = __builtin_amdgcn_readfirstlane((int)((K + kKC - 1) / kKC));
+    if (chunks > 0) {
+        fetch(0);
+        store(0);
+    }
+    __syncthreads();
+    for (int ch = 0; ch < chunks; ++ch) {
+        const int cur = ch & 1;
+        const bool more = ch + 1 < chunks;
+        if (more) fetch((int64_t)(ch + 1) * kKC);
+        const T *pr = lds[cur][0];
+        const T *pc = lds[cur][1];
 #pragma unroll
         for (int ks = 0; ks < kKC; ks += 4) {
             const int kr = ks + (lane >> 4);
@@ -215,6 +286,27 @@ __global__ __launch_bounds__(256) void gemm_tn_kernel(const T *__restrict__ A, i
                     acc[ti][tj] = acc_t{0, 0, 0, 0};
                 }
         }
+        if (more) store(cur ^ 1);
+        __syncthreads();
+    }
+    // Epilogue (kept in this instantiation, so the accumulators never merge
+    // across the two loop versions).  With beta != 0 all old values of C are
+    // loaded first, from in-range (clamped) addresses without branches, so
+    // the loads overlap instead of completing one by one.
+    double old[4][4][4];
+    if (beta != 0.0) {
+#pragma unroll
+        for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+            for (int tj = 0; tj < 4; ++tj)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    int64_t r = r0 + 64 * wr + 16 * ti + Mfma<T>::row(lane, v);
+                    int64_t c = c0 + 64 * wc + 16 * tj + (lane & 15);
+                    r = r < M ? r : M - 1;
+                    c = c < N ? c : N - 1;
+                    old[ti][tj][v] = C[r * ldc + c];
+                }
     }
 #pragma unroll
     for (int ti = 0; ti < 4; ++ti)
@@ -229,32 +321,82 @@ __global__ __launch_bounds__(256) void gemm_tn_kernel(const T *__restrict__ A, i
                                              : (double)acc[ti][tj][v];
                     double val = sign * sum;
                     if (r == c) val += shift;
-                    C[r * ldc + c] = beta == 0.0 ? val : fma(beta, C[r * ldc + c], val);
+                    C[r * ldc + c] = beta == 0.0 ? val : fma(beta, old[ti][tj][v], val);
                 }
             }
+}
+
+// C = beta * C + sign * A^T B + shift * I on the selected tiles, with A
+// (K x M) and B (K x N) row-major panels and C (M x N) row-major (ldc).
+// UPPER: M == N and only the upper triangle (r <= c) is computed (the
+// symmetric rank-K update A^T A when B == A).  A workgroup of 4 wavefronts
+// owns a kST x kST tile of C (2 x 2 waves of 64 x 64, each 4 x 4 MFMA
+// 16x16x4 tiles).  The K dimension goes through two LDS buffers in chunks
+// of kKC rows: the global loads of chunk c + 1 are issued before the MFMAs
+// of chunk c and written to the other buffer after them, so their latency
+// overlaps the matrix work and one barrier per chunk suffices.  Tiles fully
+// inside aligned panels (with K a multiple of kKC) run the unchecked loop.
+// FP32 MFMA accumulators are folded into FP64 after every chunk (FP32
+// inputs, FP64 sums).  Every workgroup reads all its inputs before it
+// writes its tile, and tiles are disjoint, so C may alias B when M <= kST
+// (used by the in-place panel solve of the Cholesky factorisation).
+template <class T, bool UPPER>
+__global__ __launch_bounds__(256) void gemm_tn_kernel(Panel<T> A, Panel<T> B, int64_t K,
+                                                      int64_t M, int64_t N, double *C, int64_t ldc,
+                                                      double beta, double sign, double shift,
+                                                      int64_t tiles_n) {
+    __shared__ T lds[2][2][kKC * kPad];
+    int64_t rb, cb;
+    if (UPPER) {
+        int64_t b = blockIdx.x, rem = tiles_n;
+        rb = 0;
+        while (b >= rem) {
+            b -= rem;
+            ++rb;
+            --rem;
+        }
+        cb = rb + b;
+    } else {
+        rb = blockIdx.x / tiles_n;
+        cb = blockIdx.x - rb * tiles_n;
+    }
+    const int64_t r0 = rb * kST, c0 = cb * kST;
+    const bool fast = A.vec && B.vec && r0 + kST <= M && c0 + kST <= N && K % kKC == 0;
+    if (fast)
+        tile_k_loop<T, true, UPPER>(A, B, K, r0, c0, lds, M, N, C, ldc, beta, sign, shift);
+    else
+        tile_k_loop<T, false, UPPER>(A, B, K, r0, c0, lds, M, N, C, ldc, beta, sign, shift);
+}
+
+template <class T>
+int launch_gemm(const T *A, int64_t lda, const T *B, int64_t ldb, int64_t K, int64_t M,
+                int64_t N, double *C, int64_t ldc, double beta, double sign, double shift,
+                bool upper, hipStream_t s) {
+    if (M <= 0 || N <= 0) return HMX_OK;
+    const int64_t tm = (M + kST - 1) / kST, tn = (N + kST - 1) / kST;
+    const Panel<T> pa = make_panel(A, lda, K, M), pb = make_panel(B, ldb, K, N);
+    if (upper) {
+        hipLaunchKernelGGL((gemm_tn_kernel<T, true>), dim3((unsigned)(tn * (tn + 1) / 2)),
+                           dim3(256), 0, s, pa, pb, K, M, N, C, ldc, beta, sign, shift, tn);
+        HMX_LAUNCHED("gemm_tn_kernel<upper>");
+    } else {
+        hipLaunchKernelGGL((gemm_tn_kernel<T, false>), dim3((unsigned)(tm * tn)), dim3(256), 0,
+                           s, pa, pb, K, M, N, C, ldc, beta, sign, shift, tn);
+        HMX_LAUNCHED("gemm_tn_kernel");
+    }
+    return HMX_OK;
 }
 
 template <class T>
 int launch_syrk(const T *X, int64_t ldx, int64_t K, int64_t m, double *C, int64_t ldc, double beta,
                 double sign, double shift, hipStream_t s) {
-    if (m <= 0) return HMX_OK;
-    const int64_t tiles = (m + kST - 1) / kST;
-    const int64_t blocks = tiles * (tiles + 1) / 2;
-    hipLaunchKernelGGL((gemm_tn_kernel<T, true>), dim3((unsigned)blocks), dim3(256), 0, s, X, ldx, X,
-                       ldx, K, m, m, C, ldc, beta, sign, shift, tiles);
-    HMX_LAUNCHED("gemm_tn_kernel<upper>");
-    return HMX_OK;
+    return launch_gemm<T>(X, ldx, X, ldx, K, m, m, C, ldc, beta, sign, shift, true, s);
 }
 
 int launch_gemm_tn(const double *A, int64_t lda, const double *B, int64_t ldb, int64_t K,
                    int64_t M, int64_t N, double *C, int64_t ldc, double beta, double sign,
                    hipStream_t s) {
-    if (M <= 0 || N <= 0) return HMX_OK;
-    const int64_t tm = (M + kST - 1) / kST, tn = (N + kST - 1) / kST;
-    hipLaunchKernelGGL((gemm_tn_kernel<double, false>), dim3((unsigned)(tm * tn)), dim3(256), 0, s,
-                       A, lda, B, ldb, K, M, N, C, ldc, beta, sign, 0.0, tn);
-    HMX_LAUNCHED("gemm_tn_kernel");
-    return HMX_OK;
+    return launch_gemm<double>(A, lda, B, ldb, K, M, N, C, ldc, beta, sign, 0.0, false, s);
 }
 
 // ---------------------------------------------------------------- Cholesky
@@ -328,63 +470,97 @@ __global__ __launch_bounds__(1024) void potrf_diag_kernel(double *__restrict__ D
 }
 
 // ------------------------------------------------------------------ solves
-// One step of the blocked forward substitution U^T y = r for the block at
-// rows [p, p + nb): every workgroup forms y_b = U_bb^-T r_b from DinvT (the
-// 1-based workgroup 0 stores it), then updates its slice of the remaining
-// right-hand side r[c] -= sum_k U[p + k][c] y_b[k], c in [p + nb, m).
-__global__ __launch_bounds__(256) void trsv_fwd_step_kernel(const double *__restrict__ U,
-                                                            int64_t ld, int64_t m, int64_t p,
-                                                            int nb,
-                                                            const double *__restrict__ DinvT,
-                                                            double *__restrict__ r,
-                                                            double *__restrict__ y) {
-    __shared__ double yb[kNB];
-    __shared__ double rb[kNB];
-    if (threadIdx.x < nb) rb[threadIdx.x] = r[p + threadIdx.x];
+// Diagonal-block step of the blocked substitutions: out_b = M v_b for the
+// nb x nb triangular inverse M of one diagonal block, with
+//   forward  (U^T y = r):  out[k] = sum_{l <= k} Dinv[l][k] v[l]  (M = Dinv^T)
+//   backward (U x = y):    out[t] = sum_{l >= t} DinvT[l][t] v[l]  (M = Dinv)
+// (Dinv = U_bb^-1 and DinvT its transpose, both row-major kNB x kNB).  One
+// workgroup of 1024 threads: 8 groups of 16 terms per output, consecutive
+// outputs on consecutive lanes (coalesced), partial sums added in order.
+__global__ __launch_bounds__(1024) void trsv_diag_kernel(const double *__restrict__ Minv, int nb,
+                                                        bool lower_terms,
+                                                        const double *__restrict__ v,
+                                                        double *__restrict__ out) {
+    __shared__ double vb[kNB];
+    __shared__ double part[8][kNB];
+    const int t = threadIdx.x & (kNB - 1), g = threadIdx.x >> 7;
+    if (threadIdx.x < nb) vb[threadIdx.x] = v[threadIdx.x];
+    __syncthreads();
+    double acc = 0.0;
+    if (t < nb) {
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int l = 16 * g + q;
+            const bool in = l < nb && (lower_terms ? l <= t : l >= t);
+            acc = in ? fma(Minv[l * kNB + t], vb[l], acc) : acc;
+        }
+    }
+    part[g][t] = acc;
     __syncthreads();
     if (threadIdx.x < nb) {
-        // y_b[k] = sum_l Dinv[l][k] r_b[l] = sum_l DinvT[k][l] r_b[l]
-        double acc = 0.0;
-        const double *row = DinvT + threadIdx.x * kNB;
-        for (int l = 0; l <= (int)threadIdx.x; ++l) acc = fma(row[l], rb[l], acc);
-        yb[threadIdx.x] = acc;
-        if (blockIdx.x == 0) y[p + threadIdx.x] = acc;
-    }
-    __syncthreads();
-    const int64_t base = p + nb;
-    for (int64_t c = base + (int64_t)blockIdx.x * 256 + threadIdx.x; c < m;
-         c += (int64_t)gridDim.x * 256) {
-        double v = r[c];
-        for (int k = 0; k < nb; ++k) v = fma(-U[(p + k) * ld + c], yb[k], v);
-        r[c] = v;
+        double sum = 0.0;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) sum += part[q][threadIdx.x];
+        out[threadIdx.x] = sum;
     }
 }
 
-// One step of the blocked backward substitution U x = y for the block at
-// rows [p, p + nb): x_b = U_bb^-1 y_b, then y[i] -= sum_c U[i][p + c] x_b[c]
-// for i in [0, p).
-__global__ __launch_bounds__(256) void trsv_bwd_step_kernel(const double *__restrict__ U,
-                                                            int64_t ld, int64_t p, int nb,
-                                                            const double *__restrict__ Dinv,
-                                                            double *__restrict__ y,
-                                                            double *__restrict__ x) {
-    __shared__ double xb[kNB];
+// Forward substitution U^T y = r, update after the block at rows [p, p + nb)
+// (y_b already solved): r[c] -= sum_k U[p + k][c] y_b[k] for c in [p + nb, m),
+// 64 columns per workgroup, the 4 wavefronts taking 32 rows k each (64
+// consecutive columns per load, coalesced), partial sums added in order.
+__global__ __launch_bounds__(256) void trsv_fwd_update_kernel(const double *__restrict__ U,
+                                                              int64_t ld, int64_t m, int64_t p,
+                                                              int nb,
+                                                              const double *__restrict__ yb_g,
+                                                              double *__restrict__ r) {
     __shared__ double yb[kNB];
-    if (threadIdx.x < nb) yb[threadIdx.x] = y[p + threadIdx.x];
+    __shared__ double part[4][64];
+    if (threadIdx.x < nb) yb[threadIdx.x] = yb_g[threadIdx.x];
     __syncthreads();
-    if (threadIdx.x < nb) {
-        double acc = 0.0;
-        const double *row = Dinv + threadIdx.x * kNB;
-        for (int l = threadIdx.x; l < nb; ++l) acc = fma(row[l], yb[l], acc);
-        xb[threadIdx.x] = acc;
-        if (blockIdx.x == 0) x[p + threadIdx.x] = acc;
+    const int lane = threadIdx.x & 63, g = threadIdx.x >> 6;
+    const int64_t c = p + nb + (int64_t)blockIdx.x * 64 + lane;
+    double v = 0.0;
+    if (c < m) {
+        const int k1 = (g + 1) * 32 < nb ? (g + 1) * 32 : nb;
+#pragma unroll 8
+        for (int k = g * 32; k < k1; ++k) v = fma(U[(p + k) * ld + c], yb[k], v);
     }
+    part[g][lane] = v;
     __syncthreads();
-    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < p; i += (int64_t)gridDim.x * 256) {
-        double v = y[i];
-        const double *row = U + i * ld + p;
-        for (int c = 0; c < nb; ++c) v = fma(-row[c], xb[c], v);
-        y[i] = v;
+    if (g == 0 && c < m) r[c] -= ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane];
+}
+
+// Backward substitution U x = y, update after the block at rows [p, p + nb)
+// (x_b already solved): y[i] -= sum_c U[i][p + c] x_b[c] for i in [0, p).
+// Each wavefront takes 16 rows, 4 at a time; its 64 lanes read a row's nb
+// values coalesced and a butterfly sum combines them.
+__global__ __launch_bounds__(256) void trsv_bwd_update_kernel(const double *__restrict__ U,
+                                                              int64_t ld, int64_t p, int nb,
+                                                              const double *__restrict__ xb_g,
+                                                              double *__restrict__ y) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const double x0 = lane < nb ? xb_g[lane] : 0.0;
+    const double x1 = lane + 64 < nb ? xb_g[lane + 64] : 0.0;
+    for (int q = 0; q < 16; q += 4) {
+        const int64_t i0 = (int64_t)blockIdx.x * 64 + w * 16 + q;
+        double v[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int64_t i = i0 + t;
+            const double *row = U + i * ld + p;
+            double a = (i < p && lane < nb) ? row[lane] * x0 : 0.0;
+            if (i < p && lane + 64 < nb) a = fma(row[lane + 64], x1, a);
+            v[t] = a;
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+            for (int t = 0; t < 4; ++t) v[t] += __shfl_xor(v[t], off, 64);
+        if (lane < 4) {
+            const double d = lane == 0 ? v[0] : lane == 1 ? v[1] : lane == 2 ? v[2] : v[3];
+            if (i0 + lane < p) y[i0 + lane] -= d;
+        }
     }
 }
 
@@ -527,6 +703,12 @@ __global__ __launch_bounds__(256) void colsumsq_kernel(const T *__restrict__ X, 
     out[j] = acc;
 }
 
+// A[j][j] += tau for j < m (row-major, leading dimension m).
+__global__ void add_diag_kernel(double *__restrict__ A, int64_t m, double tau) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < m) A[j * m + j] += tau;
+}
+
 __global__ void zero_int_kernel(int *p) {
     if (threadIdx.x == 0) *p = 0;
 }
@@ -571,24 +753,51 @@ struct Fit {
         return HMX_OK;
     }
 
+    // Blocked right-looking factorisation A = U^T U (upper, in place) in two
+    // levels: within an outer block of kOuter rows, kNB-row steps factor the
+    // diagonal block, solve their row panel up to column m and update only
+    // the remaining rows of the outer block; then one rank-kOuter update
+    // (K = kOuter, the regime where the MFMA kernel is efficient) brings the
+    // trailing matrix up to date.
     int cholesky(int *pivot) {
         hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(1), 0, s, dinfo);
         HMX_LAUNCHED("zero_int_kernel");
-        for (int64_t p = 0; p < m; p += kNB) {
-            const int nb = (int)(m - p < kNB ? m - p : kNB);
-            double *D = A + p * m + p;
-            const int64_t blk = p / kNB;
-            hipLaunchKernelGGL(potrf_diag_kernel, dim3(1), dim3(1024), 0, s, D, m, nb, p,
-                               Dinv + blk * kNB * kNB, DinvT + blk * kNB * kNB, dinfo);
-            HMX_LAUNCHED("potrf_diag_kernel");
-            const int64_t rest = m - p - nb;
-            if (rest <= 0) break;
-            // U12 = U11^-T A12 = (U11^-1)^T A12, in place.
-            int st = launch_gemm_tn(Dinv + blk * kNB * kNB, kNB, D + nb, m, nb, nb, rest, D + nb, m,
-                                    0.0, 1.0, s);
-            if (st) return st;
-            st = launch_syrk<double>(D + nb, m, nb, rest, D + nb * m + nb, m, 1.0, -1.0, 0.0, s);
-            if (st) return st;
+        for (int64_t P = 0; P < m; P += kOuter) {
+            const int64_t Pend = P + kOuter < m ? P + kOuter : m;
+            for (int64_t p = P; p < Pend; p += kNB) {
+                const int nb = (int)(m - p < kNB ? m - p : kNB);
+                double *D = A + p * m + p;
+                const int64_t blk = p / kNB;
+                hipLaunchKernelGGL(potrf_diag_kernel, dim3(1), dim3(1024), 0, s, D, m, nb, p,
+                                   Dinv + blk * kNB * kNB, DinvT + blk * kNB * kNB, dinfo);
+                HMX_LAUNCHED("potrf_diag_kernel");
+                const int64_t rest = m - p - nb;
+                if (rest <= 0) break;
+                // U12 = U11^-T A12 = (U11^-1)^T A12, in place, up to column m.
+                int st = launch_gemm_tn(Dinv + blk * kNB * kNB, kNB, D + nb, m, nb, nb, rest,
+                                        D + nb, m, 0.0, 1.0, s);
+                if (st) return st;
+                // Rows [p + nb, Pend) of the outer block, columns [p + nb, m)
+                // (the few entries below the diagonal that this rectangle
+                // covers are never read).
+                const int64_t rows_in = Pend - p - nb;
+                if (rows_in > 0) {
+                    st = launch_gemm<double>(D + nb, m, D + nb, m, nb, rows_in, rest,
+                                             D + nb * m + nb, m, 1.0, -1.0, 0.0, false, s);
+                    if (st) return st;
+                }
+                if (blk % 16 == 15) {  // stop early once a pivot has failed
+                    HMX_HIP(hipMemcpyAsync(pivot, dinfo, sizeof(int), hipMemcpyDeviceToHost, s));
+                    HMX_HIP(hipStreamSynchronize(s));
+                    if (*pivot) return HMX_OK;
+                }
+            }
+            if (Pend < m) {
+                const int64_t K = Pend - P, rest = m - Pend;
+                int st = launch_syrk<double>(A + P * m + Pend, m, K, rest, A + Pend * m + Pend, m,
+                                             1.0, -1.0, 0.0, s);
+                if (st) return st;
+            }
         }
         HMX_HIP(hipMemcpyAsync(pivot, dinfo, sizeof(int), hipMemcpyDeviceToHost, s));
         HMX_HIP(hipStreamSynchronize(s));
@@ -596,22 +805,32 @@ struct Fit {
     }
 
     // x = A^-1 r with the factor; r is overwritten (work), y is scratch.
+    // Per kNB block: the diagonal-block product writes the block's solution,
+    // then the update kernel applies it to the rest of the right-hand side.
     int solve(double *r, double *y, double *x) {
         for (int64_t p = 0; p < m; p += kNB) {
             const int nb = (int)(m - p < kNB ? m - p : kNB);
             const int64_t rest = m - p - nb;
-            const unsigned g = blocks_for(rest > 0 ? rest : 1, 1024);
-            hipLaunchKernelGGL(trsv_fwd_step_kernel, dim3(g), dim3(256), 0, s, A, m, m, p, nb,
-                               DinvT + (p / kNB) * kNB * kNB, r, y);
-            HMX_LAUNCHED("trsv_fwd_step_kernel");
+            hipLaunchKernelGGL(trsv_diag_kernel, dim3(1), dim3(1024), 0, s,
+                               Dinv + (p / kNB) * kNB * kNB, nb, true, r + p, y + p);
+            HMX_LAUNCHED("trsv_diag_kernel");
+            if (rest > 0) {
+                hipLaunchKernelGGL(trsv_fwd_update_kernel, dim3((unsigned)((rest + 63) / 64)),
+                                   dim3(256), 0, s, A, m, m, p, nb, y + p, r);
+                HMX_LAUNCHED("trsv_fwd_update_kernel");
+            }
         }
         const int64_t last = ((m - 1) / kNB) * kNB;
         for (int64_t p = last; p >= 0; p -= kNB) {
             const int nb = (int)(m - p < kNB ? m - p : kNB);
-            const unsigned g = blocks_for(p > 0 ? p : 1, 1024);
-            hipLaunchKernelGGL(trsv_bwd_step_kernel, dim3(g), dim3(256), 0, s, A, m, p, nb,
-                               Dinv + (p / kNB) * kNB * kNB, y, x);
-            HMX_LAUNCHED("trsv_bwd_step_kernel");
+            hipLaunchKernelGGL(trsv_diag_kernel, dim3(1), dim3(1024), 0, s,
+                               DinvT + (p / kNB) * kNB * kNB, nb, false, y + p, x + p);
+            HMX_LAUNCHED("trsv_diag_kernel");
+            if (p > 0) {
+                hipLaunchKernelGGL(trsv_bwd_update_kernel, dim3((unsigned)((p + 63) / 64)),
+                                   dim3(256), 0, s, A, m, p, nb, x + p, y);
+                HMX_LAUNCHED("trsv_bwd_update_kernel");
+            }
         }
         return HMX_OK;
     }
@@ -659,31 +878,13 @@ int hmx_gemm_tn(int dtype, const void *A, int64_t lda, const void *B, int64_t ld
     if (M == 0 || N == 0) return HMX_OK;
     if (!A || !B || !C) return fail(HMX_ERR_ARG, "NULL array");
     hipStream_t s = (hipStream_t)stream;
-    const int64_t tm = (M + kST - 1) / kST, tn = (N + kST - 1) / kST;
-    const unsigned blocks = (unsigned)(upper ? tn * (tn + 1) / 2 : tm * tn);
-    if (dtype == HMX_F64) {
-        if (upper)
-            hipLaunchKernelGGL((gemm_tn_kernel<double, true>), dim3(blocks), dim3(256), 0, s,
-                               (const double *)A, lda, (const double *)B, ldb, K, M, N, C, ldc,
-                               beta, sign, shift, tn);
-        else
-            hipLaunchKernelGGL((gemm_tn_kernel<double, false>), dim3(blocks), dim3(256), 0, s,
-                               (const double *)A, lda, (const double *)B, ldb, K, M, N, C, ldc,
-                               beta, sign, shift, tn);
-    } else if (dtype == HMX_F32) {
-        if (upper)
-            hipLaunchKernelGGL((gemm_tn_kernel<float, true>), dim3(blocks), dim3(256), 0, s,
-                               (const float *)A, lda, (const float *)B, ldb, K, M, N, C, ldc, beta,
-                               sign, shift, tn);
-        else
-            hipLaunchKernelGGL((gemm_tn_kernel<float, false>), dim3(blocks), dim3(256), 0, s,
-                               (const float *)A, lda, (const float *)B, ldb, K, M, N, C, ldc, beta,
-                               sign, shift, tn);
-    } else {
-        return fail(HMX_ERR_ARG, "invalid dtype %d", dtype);
-    }
-    HMX_LAUNCHED("gemm_tn_kernel");
-    return HMX_OK;
+    if (dtype == HMX_F64)
+        return launch_gemm<double>((const double *)A, lda, (const double *)B, ldb, K, M, N, C, ldc,
+                                   beta, sign, shift, upper != 0, s);
+    if (dtype == HMX_F32)
+        return launch_gemm<float>((const float *)A, lda, (const float *)B, ldb, K, M, N, C, ldc,
+                                  beta, sign, shift, upper != 0, s);
+    return fail(HMX_ERR_ARG, "invalid dtype %d", dtype);
 }
 
 int hmx_cholesky_workspace_bytes(int64_t m, size_t *bytes) {
@@ -737,7 +938,8 @@ int hmx_eqs_fit_workspace_bytes(int64_t n_data, int64_t n_src, int precision_mod
     const size_t m = (size_t)n_src, n = (size_t)n_data;
     const size_t nblk = (m + kNB - 1) / kNB;
     const size_t x_elem = precision_mode == HMX_FIT_FP32_MFMA ? sizeof(float) : sizeof(double);
-    *bytes = m * m * sizeof(double)                      // A, then its factor U
+    const size_t copies = precision_mode == HMX_FIT_FP32_MFMA ? 2 : 1;
+    *bytes = copies * m * m * sizeof(double)             // A, then its factor U (+ a copy of A)
              + n * m * x_elem                            // X = W^1/2 J S
              + 2 * nblk * kNB * kNB * sizeof(double)     // inverses of diagonal blocks
              + (9 * m + n + 256) * sizeof(double) + 256;  // 9 m-vectors, n-vector, partials, info
@@ -784,6 +986,11 @@ int hmx_eqs_fit(const double *easting, const double *northing, const double *upw
     f.s = s;
     f.A = reinterpret_cast<double *>(ws);
     ws += (size_t)m * m * sizeof(double);
+    double *A_copy = nullptr;  // mixed mode: the unshifted normal matrix, for retries
+    if (mixed) {
+        A_copy = reinterpret_cast<double *>(ws);
+        ws += (size_t)m * m * sizeof(double);
+    }
     void *X = ws;
     ws += (size_t)nd * m * (mixed ? sizeof(float) : sizeof(double));
     f.Dinv = reinterpret_cast<double *>(ws);
@@ -837,13 +1044,22 @@ int hmx_eqs_fit(const double *easting, const double *northing, const double *upw
     };
     int pivot = 0, attempts = 0;
     for (;;) {
-        if (mixed)
+        if (mixed && attempts > 0) {  // restore the normal matrix, add the shift
+            HMX_HIP(hipMemcpyAsync(f.A, A_copy, (size_t)m * m * sizeof(double),
+                                   hipMemcpyDeviceToDevice, s));
+            hipLaunchKernelGGL(add_diag_kernel, dim3(gm), dim3(256), 0, s, f.A, m, tau);
+            HMX_LAUNCHED("add_diag_kernel");
+        } else if (mixed) {
             st = launch_syrk<float>(static_cast<const float *>(X), m, nd, m, f.A, m, 0.0, 1.0,
-                                    damping + tau, s);
-        else
+                                    damping, s);
+            if (st) return st;
+            HMX_HIP(hipMemcpyAsync(A_copy, f.A, (size_t)m * m * sizeof(double),
+                                   hipMemcpyDeviceToDevice, s));
+        } else {
             st = launch_syrk<double>(static_cast<const double *>(X), m, nd, m, f.A, m, 0.0, 1.0,
                                      damping + tau, s);
-        if (st) return st;
+            if (st) return st;
+        }
         st = f.cholesky(&pivot);
         if (st) return st;
         if (pivot == 0) break;

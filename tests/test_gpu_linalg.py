@@ -127,8 +127,10 @@ def test_syrk_upper(env, dtype, m):
     assert np.all(got[il] == 7.0)  # lower triangle untouched
 
 
-@pytest.mark.parametrize("m", [5, 128, 129, 300, 700])
+@pytest.mark.parametrize("m", [5, 128, 129, 300, 700, 1500, 2100])
 def test_cholesky_solve(env, m):
+    """Sizes below, across and beyond the 1024-row outer block (the rank-1024
+    trailing update runs for m > 1024)."""
     torch, _lib, L = env
     rng = np.random.default_rng(m)
     X = rng.normal(size=(m + 20, m))
@@ -149,11 +151,11 @@ def test_cholesky_solve(env, m):
     npt.assert_allclose(U.T @ U, A, atol=1e-10 * np.abs(A).max())
 
 
-def test_cholesky_reports_indefinite(env):
+@pytest.mark.parametrize("m,bad", [(200, 150), (1300, 1200)])
+def test_cholesky_reports_indefinite(env, m, bad):
     torch, _lib, L = env
-    m = 200
     A = np.eye(m)
-    A[150, 150] = -1.0
+    A[bad, bad] = -1.0
     nbytes = ctypes.c_size_t(0)
     _lib.check(L.hmx_cholesky_workspace_bytes(m, ctypes.byref(nbytes)))
     ws = torch.empty(nbytes.value, dtype=torch.uint8, device="cuda")
@@ -161,4 +163,4 @@ def test_cholesky_reports_indefinite(env):
     info = ctypes.c_int(-1)
     _lib.check(L.hmx_cholesky_solve(_lib.ptr(dA), m, _lib.ptr(dx), ctypes.byref(info), _lib.ptr(ws),
                                     nbytes.value, None))
-    assert info.value == 151
+    assert info.value == bad + 1
