@@ -166,10 +166,11 @@ def bench_eqs(args):
     out = {"n_data": coords[0].size}
     preds = {}
     small = tuple(c[:2000] for c in coords)
-    for mode in ("float64", "mixed"):  # untimed warm-up of both code paths
+    modes = ("float64", "mixed", "mixed-fp32")
+    for mode in modes:  # untimed warm-up of every code path
         hm.EquivalentSources(damping=w["damping"], depth=w["depth"], fit_precision=mode).fit(
             small, data[:2000])
-    for mode in ("float64", "mixed"):
+    for mode in modes:
         eqs = hm.EquivalentSources(damping=w["damping"], depth=w["depth"], fit_precision=mode)
         _, dt = _sync_time(lambda: eqs.fit(coords, data))
         pred, dtp = _sync_time(lambda: eqs.predict(w["grid"]))
@@ -177,8 +178,10 @@ def bench_eqs(args):
         out[mode] = {"fit_s": dt, "predict_s": dtp, "info": eqs.fit_info_}
         log(f"eqs {mode}: fit {dt:.2f} s, predict {dtp:.3f} s, {eqs.fit_info_}")
     ref = preds["float64"]
-    out["mixed_vs_fp64_max_rel_pred_diff"] = float(np.abs(preds["mixed"] - ref).max() / np.abs(ref).max())
-    out["speedup_mixed_vs_fp64"] = out["float64"]["fit_s"] / out["mixed"]["fit_s"]
+    for mode in modes[1:]:
+        out[mode]["max_rel_pred_diff_vs_fp64"] = float(np.abs(preds[mode] - ref).max() /
+                                                       np.abs(ref).max())
+        out[mode]["speedup_vs_fp64"] = out["float64"]["fit_s"] / out[mode]["fit_s"]
     if args.eqs_oracle:
         import oracle
 

@@ -193,9 +193,11 @@ def fit_device(coords, data, weights, points, damping, precision="float64", max_
     GPU (all inputs float64).  Returns (coefs numpy array, info dict)."""
     import ctypes
 
-    mode = _lib.HMX_FIT_FP64 if precision == "float64" else _lib.HMX_FIT_FP32_MFMA
+    modes = {"float64": _lib.HMX_FIT_FP64, "mixed": _lib.HMX_FIT_BF16_SPLIT,
+             "mixed-fp32": _lib.HMX_FIT_FP32_MFMA}
+    mode = modes[precision]
     if max_iter is None:
-        max_iter = 0 if mode == _lib.HMX_FIT_FP64 else 50
+        max_iter = 0 if mode == _lib.HMX_FIT_FP64 else 300
     c = [_dev(x, torch.float64) for x in coords]
     p = [_dev(x, torch.float64) for x in points]
     d = _dev(data, torch.float64)
@@ -227,8 +229,11 @@ class EquivalentSources(BaseEstimator):
     the GPU is always parallel), ``dtype`` (of the jacobian and predictions).
     ``fit_precision`` selects the GPU least-squares mode: ``"float64"``
     (default; FP64 normal matrix and Cholesky, like the reference's LAPACK
-    solve) or ``"mixed"`` (FP32-MFMA normal matrix as the FP64-factorised
-    preconditioner of conjugate gradients on the FP64 normal equations).
+    solve), ``"mixed"`` (normal matrix from exact bf16 MFMA products of
+    8-bit slices of the scaled jacobian, about 2^-31 relative) or
+    ``"mixed-fp32"`` (FP32-MFMA normal matrix); in the mixed modes the
+    FP64 factor of the low-precision matrix preconditions conjugate
+    gradients on the FP64 normal equations.
 
     Attributes ``points_``, ``coefs_``, ``depth_``, ``region_`` as in the
     reference; ``fit_info_`` reports the solver statistics.
@@ -256,8 +261,9 @@ class EquivalentSources(BaseEstimator):
         if depth == 0:
             msg = "Depth value cannot be zero. It should be a non-zero numeric value."
             raise ValueError(msg)
-        if fit_precision not in ("float64", "mixed"):
-            raise ValueError(f"Invalid fit_precision '{fit_precision}': 'float64' or 'mixed'.")
+        if fit_precision not in ("float64", "mixed", "mixed-fp32"):
+            raise ValueError(f"Invalid fit_precision '{fit_precision}': 'float64', 'mixed' or "
+                             "'mixed-fp32'.")
         self.damping = damping
         self.points = points
         self.depth = depth

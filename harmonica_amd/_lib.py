@@ -28,6 +28,7 @@ HMX_CARTESIAN = 0
 HMX_SPHERICAL = 1
 HMX_FIT_FP64 = 0
 HMX_FIT_FP32_MFMA = 1
+HMX_FIT_BF16_SPLIT = 2
 
 #: Field names in the reference's FIELDS order (gravity.py:37-48); bit i of a
 #: field mask selects FIELDS[i].

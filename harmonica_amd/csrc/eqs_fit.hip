@@ -31,6 +31,7 @@ namespace {
 
 typedef double double4_t __attribute__((ext_vector_type(4)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 
 constexpr int kNB = 128;    // Cholesky block size
 constexpr int kOuter = 1024;  // Cholesky outer block (K of the big trailing updates)
@@ -713,6 +714,234 @@ __global__ void zero_int_kernel(int *p) {
     if (threadIdx.x == 0) *p = 0;
 }
 
+// ------------------------------------------ split-precision normal matrix
+// Exact-product scheme for X^T X on bf16 MFMA.  Column j of
+// X = W^1/2 J S is scaled by f_j = 2^(E_j + 1), where max_k |X_kj| < 2^E_j,
+// so x = X_kj / f_j lies in (-1/2, 1/2), and cut into kSlices integer
+// slices: sigma_0 = rint(256 x), sigma_1 = rint(256 (256 x - sigma_0)), ...
+// with |sigma_t| <= 128, so that
+//   X_kj = f_j sum_t sigma_t,kj 256^-(t+1)   (error below f_j 2^-33).
+// Every sigma is an integer of at most 8 bits (exact in bf16) and every
+// product of two has at most 14 bits, so MFMA sums of up to 1024 products
+// of one order are exact in FP32.  The normal matrix
+//   (X^T X)_ij = f_i f_j sum_o 256^-(o+2) sum_{s+t=o} sum_k sigma_s,ki sigma_t,kj
+// keeps the orders o = s + t <= 3 (10 slice pairs); the dropped orders are
+// below 2^-31 of f_i f_j n.  Slices are stored k-packed: slice t, rows
+// 8 g .. 8 g + 7 of column j are the 8 consecutive bf16 at
+// ((t * n_groups + g) * m + j) * 8, one 16-byte load per MFMA operand.
+constexpr int kSlices = 4;
+constexpr int kFoldChunks = 32;  // 32 chunks of 32 rows: order-0 sums stay exact
+
+// f_j = 2^(E_j + 1) with max_k sqrt(w_k) J_kj / s_j < 2^E_j (1 if all zero).
+__global__ __launch_bounds__(256) void colscale_pow2_kernel(
+    const double *__restrict__ e, const double *__restrict__ n, const double *__restrict__ u,
+    const double *__restrict__ w, int64_t n_data, const double *__restrict__ pe,
+    const double *__restrict__ pn, const double *__restrict__ pu,
+    const double *__restrict__ scale, int64_t n_src, double *__restrict__ f) {
+    __shared__ double4_t tile[256];
+    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t jj = j < n_src ? j : n_src - 1;
+    const double se = pe[jj], sn = pn[jj], su = pu[jj];
+    double mx = 0.0;
+    for (int64_t t0 = 0; t0 < n_data; t0 += 256) {
+        const int cnt = (int)(n_data - t0 < 256 ? n_data - t0 : 256);
+        __syncthreads();
+        if (threadIdx.x < cnt) {
+            const int64_t i = t0 + threadIdx.x;
+            double4_t v;
+            v.x = e[i];
+            v.y = n[i];
+            v.z = u[i];
+            v.w = w ? sqrt(w[i]) : 1.0;
+            tile[threadIdx.x] = v;
+        }
+        __syncthreads();
+        for (int q = 0; q < cnt; ++q) {
+            const double4_t o = tile[q];
+            mx = fmax(mx, o.w * inv_dist(o.x, o.y, o.z, se, sn, su));
+        }
+    }
+    if (j < n_src) {
+        const double cmax = mx / scale[j];
+        int ex = 0;
+        (void)frexp(cmax, &ex);  // cmax < 2^ex
+        f[j] = cmax > 0.0 ? ldexp(1.0, ex + 1) : 1.0;
+    }
+}
+
+// The kSlices slices of X / f in the k-packed layout (rows >= n_data are 0).
+__global__ __launch_bounds__(256) void slice_kernel(
+    const double *__restrict__ e, const double *__restrict__ n, const double *__restrict__ u,
+    const double *__restrict__ w, int64_t n_data, const double *__restrict__ pe,
+    const double *__restrict__ pn, const double *__restrict__ pu,
+    const double *__restrict__ scale, const double *__restrict__ f, int64_t n_src,
+    int64_t n_groups, bf16x8_t *__restrict__ S) {
+    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (j >= n_src) return;
+    const double se = pe[j], sn = pn[j], su = pu[j];
+    const double inv = 1.0 / (scale[j] * f[j]);
+    for (int64_t g = blockIdx.y; g < n_groups; g += gridDim.y) {
+        bf16x8_t out[kSlices];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int64_t k = 8 * g + q;
+            double x = 0.0;
+            if (k < n_data) {
+                const double sw = w ? sqrt(w[k]) : 1.0;
+                x = sw * inv_dist(e[k], n[k], u[k], se, sn, su) * inv;
+            }
+#pragma unroll
+            for (int t = 0; t < kSlices; ++t) {
+                const double y = 256.0 * x;
+                const double sig = rint(y);
+                x = y - sig;
+                out[t][q] = (__bf16)(float)sig;
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < kSlices; ++t) S[(t * n_groups + g) * n_src + j] = out[t];
+    }
+}
+
+// Orders O0..O1 of the split-precision normal matrix on the upper-triangle
+// tiles: C = beta C + f_r f_c sum_o 256^-(o+2) sum_{s+t=o} S_s^T S_t
+// (+ shift on the diagonal).  A workgroup of 8 wavefronts owns a kST x kST
+// tile (2 x 4 waves of 64 x 32, 4 x 2 MFMA 16x16x32 bf16 tiles each) and
+// one FP32 accumulator set per order, folded into FP64 every kFoldChunks
+// chunks of 32 rows.  Slices 0..O1 of both panels go through two LDS
+// buffers with the loads of the next chunk in flight during the MFMAs.
+template <int O0, int O1>
+__global__ __launch_bounds__(512) void gram_bf16_kernel(const bf16x8_t *__restrict__ S,
+                                                        int64_t n_groups, int64_t m,
+                                                        const double *__restrict__ f,
+                                                        double *__restrict__ C, int64_t ldc,
+                                                        double beta, double shift,
+                                                        int64_t tiles_n) {
+    constexpr int kNO = O1 - O0 + 1;  // orders accumulated here
+    constexpr int kNS = O1 + 1;       // slices needed on each side
+    __shared__ bf16x8_t lds[2][2][kNS][4][kST];
+    int64_t b = blockIdx.x, rem = tiles_n, rb = 0;
+    while (b >= rem) {
+        b -= rem;
+        ++rb;
+        --rem;
+    }
+    const int64_t cb = rb + b;
+    const int64_t r0 = rb * kST, c0 = cb * kST;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wr = wave >> 2, wc = wave & 3;
+    const bf16x8_t zero8 = {};
+    float4_t acc[kNO][4][2];
+    double4_t dacc[4][2];
+#pragma unroll
+    for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < 2; ++tj) dacc[ti][tj] = double4_t{0, 0, 0, 0};
+    // staging: per side and slice, 4 k-groups x kST columns = 1 unit/thread
+    const int kg = threadIdx.x >> 7, col = threadIdx.x & (kST - 1);
+    bf16x8_t pre[2][kNS];
+    auto fetch = [&](int ch) {
+        const int64_t g = 4 * (int64_t)ch + kg;
+#pragma unroll
+        for (int side = 0; side < 2; ++side) {
+            const int64_t gc = (side ? c0 : r0) + col;
+#pragma unroll
+            for (int t = 0; t < kNS; ++t)
+                pre[side][t] = gc < m ? S[((int64_t)t * n_groups + g) * m + gc] : zero8;
+        }
+    };
+    auto store = [&](int buf) {
+#pragma unroll
+        for (int side = 0; side < 2; ++side)
+#pragma unroll
+            for (int t = 0; t < kNS; ++t) lds[buf][side][t][kg][col] = pre[side][t];
+    };
+    const int chunks = __builtin_amdgcn_readfirstlane((int)(n_groups / 4));
+    if (chunks > 0) {
+        fetch(0);
+        store(0);
+    }
+    __syncthreads();
+    for (int g0 = 0; g0 < chunks; g0 += kFoldChunks) {
+        const int g1 = g0 + kFoldChunks < chunks ? g0 + kFoldChunks : chunks;
+        // accumulators born in the matrix unit, so they stay in its registers
+#pragma unroll
+        for (int o = 0; o < kNO; ++o)
+#pragma unroll
+            for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+                for (int tj = 0; tj < 2; ++tj)
+                    acc[o][ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                        zero8, zero8, float4_t{0, 0, 0, 0}, 0, 0, 0);
+        for (int ch = g0; ch < g1; ++ch) {
+            const int cur = ch & 1;
+            const bool more = ch + 1 < chunks;
+            if (more) fetch(ch + 1);
+#pragma unroll
+            for (int o = 0; o < kNO; ++o)
+#pragma unroll
+                for (int sa = 0; sa <= O0 + o; ++sa) {
+                    const int sb = O0 + o - sa;
+                    bf16x8_t a[4], bb[2];
+#pragma unroll
+                    for (int t = 0; t < 4; ++t)
+                        a[t] = lds[cur][0][sa][lane >> 4][64 * wr + 16 * t + (lane & 15)];
+#pragma unroll
+                    for (int t = 0; t < 2; ++t)
+                        bb[t] = lds[cur][1][sb][lane >> 4][32 * wc + 16 * t + (lane & 15)];
+#pragma unroll
+                    for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+                        for (int tj = 0; tj < 2; ++tj)
+                            acc[o][ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                                a[ti], bb[tj], acc[o][ti][tj], 0, 0, 0);
+                }
+            if (more) store(cur ^ 1);
+            __syncthreads();
+        }
+#pragma unroll
+        for (int o = 0; o < kNO; ++o) {
+            const double sc = ldexp(1.0, -8 * (O0 + o + 2));
+#pragma unroll
+            for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+                for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v)
+                        dacc[ti][tj][v] = fma((double)acc[o][ti][tj][v], sc, dacc[ti][tj][v]);
+        }
+    }
+    double old[4][2][4];
+    if (beta != 0.0) {
+#pragma unroll
+        for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+            for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    int64_t r = r0 + 64 * wr + 16 * ti + 4 * (lane >> 4) + v;
+                    int64_t c = c0 + 32 * wc + 16 * tj + (lane & 15);
+                    r = r < m ? r : m - 1;
+                    c = c < m ? c : m - 1;
+                    old[ti][tj][v] = C[r * ldc + c];
+                }
+    }
+#pragma unroll
+    for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int64_t r = r0 + 64 * wr + 16 * ti + 4 * (lane >> 4) + v;
+                const int64_t c = c0 + 32 * wc + 16 * tj + (lane & 15);
+                if (r < m && c < m && r <= c) {
+                    double val = dacc[ti][tj][v] * f[r] * f[c];
+                    if (r == c) val += shift;
+                    C[r * ldc + c] = beta == 0.0 ? val : old[ti][tj][v] + val;
+                }
+            }
+}
+
 // ----------------------------------------------------------- host driver
 inline unsigned blocks_for(int64_t n, int per = 256) {
     const int64_t b = (n + per - 1) / per;
@@ -845,9 +1074,18 @@ __global__ void mfma_probe_kernel(int dtype, const double *__restrict__ a,
         double4_t c = {0, 0, 0, 0};
         c = Mfma<double>::step(a[l], b[l], c);
         for (int v = 0; v < 4; ++v) out[4 * l + v] = c[v];
-    } else {
+    } else if (dtype == HMX_F32) {
         float4_t c = {0, 0, 0, 0};
         c = Mfma<float>::step((float)a[l], (float)b[l], c);
+        for (int v = 0; v < 4; ++v) out[4 * l + v] = (double)c[v];
+    } else {
+        bf16x8_t va, vb;
+        for (int e = 0; e < 8; ++e) {
+            va[e] = (__bf16)(float)a[8 * l + e];
+            vb[e] = (__bf16)(float)b[8 * l + e];
+        }
+        float4_t c = {0, 0, 0, 0};
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va, vb, c, 0, 0, 0);
         for (int v = 0; v < 4; ++v) out[4 * l + v] = (double)c[v];
     }
 }
@@ -860,7 +1098,8 @@ extern "C" {
 int hmx_mfma_probe(int dtype, const double *a, const double *b, double *out, void *stream) {
     using namespace hmx;
     clear_error();
-    if (dtype != HMX_F64 && dtype != HMX_F32) return fail(HMX_ERR_ARG, "invalid dtype %d", dtype);
+    if (dtype != HMX_F64 && dtype != HMX_F32 && dtype != HMX_BF16)
+        return fail(HMX_ERR_ARG, "invalid dtype %d", dtype);
     if (!a || !b || !out) return fail(HMX_ERR_ARG, "NULL array");
     hipLaunchKernelGGL(mfma_probe_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, dtype, a, b, out);
     HMX_LAUNCHED("mfma_probe_kernel");
@@ -933,16 +1172,19 @@ int hmx_eqs_fit_workspace_bytes(int64_t n_data, int64_t n_src, int precision_mod
     using namespace hmx;
     if (!bytes) return fail(HMX_ERR_ARG, "bytes is NULL");
     if (n_data <= 0 || n_src <= 0) return fail(HMX_ERR_ARG, "empty problem");
-    if (precision_mode != HMX_FIT_FP64 && precision_mode != HMX_FIT_FP32_MFMA)
+    if (precision_mode != HMX_FIT_FP64 && precision_mode != HMX_FIT_FP32_MFMA &&
+        precision_mode != HMX_FIT_BF16_SPLIT)
         return fail(HMX_ERR_ARG, "invalid precision mode %d", precision_mode);
     const size_t m = (size_t)n_src, n = (size_t)n_data;
     const size_t nblk = (m + kNB - 1) / kNB;
-    const size_t x_elem = precision_mode == HMX_FIT_FP32_MFMA ? sizeof(float) : sizeof(double);
-    const size_t copies = precision_mode == HMX_FIT_FP32_MFMA ? 2 : 1;
+    const size_t n_pad = (n + 31) / 32 * 32;
+    size_t x_bytes = n * m * sizeof(double);                        // X = W^1/2 J S
+    if (precision_mode == HMX_FIT_FP32_MFMA) x_bytes = n * m * sizeof(float);
+    if (precision_mode == HMX_FIT_BF16_SPLIT) x_bytes = kSlices * n_pad * m * 2;  // slices
+    const size_t copies = precision_mode == HMX_FIT_FP64 ? 1 : 2;
     *bytes = copies * m * m * sizeof(double)             // A, then its factor U (+ a copy of A)
-             + n * m * x_elem                            // X = W^1/2 J S
-             + 2 * nblk * kNB * kNB * sizeof(double)     // inverses of diagonal blocks
-             + (9 * m + n + 256) * sizeof(double) + 256;  // 9 m-vectors, n-vector, partials, info
+             + x_bytes + 2 * nblk * kNB * kNB * sizeof(double)  // inverses of diagonal blocks
+             + (10 * m + n + 256) * sizeof(double) + 256;  // 10 m-vectors, n-vector, partials, info
     return HMX_OK;
 }
 
@@ -968,8 +1210,11 @@ int hmx_eqs_fit(const double *easting, const double *northing, const double *upw
         return fail(HMX_ERR_ARG, "workspace too small: %zu < %zu bytes", workspace_bytes, need);
     hipStream_t s = (hipStream_t)stream;
     const int64_t m = n_src, nd = n_data;
-    const bool mixed = precision_mode == HMX_FIT_FP32_MFMA;
+    // mixed: a low-precision normal matrix preconditions CG on the FP64 one
+    const bool mixed = precision_mode != HMX_FIT_FP64;
+    const bool split = precision_mode == HMX_FIT_BF16_SPLIT;
     const size_t nblk = (size_t)((m + kNB - 1) / kNB);
+    const int64_t n_groups = (nd + 31) / 32 * 4;  // k-groups of 8 rows, padded to 32 rows
 
     char *ws = static_cast<char *>(workspace);
     Fit f;
@@ -992,14 +1237,17 @@ int hmx_eqs_fit(const double *easting, const double *northing, const double *upw
         ws += (size_t)m * m * sizeof(double);
     }
     void *X = ws;
-    ws += (size_t)nd * m * (mixed ? sizeof(float) : sizeof(double));
+    if (split)
+        ws += (size_t)kSlices * n_groups * 8 * m * 2;
+    else
+        ws += (size_t)nd * m * (mixed ? sizeof(float) : sizeof(double));
     f.Dinv = reinterpret_cast<double *>(ws);
     f.DinvT = f.Dinv + nblk * kNB * kNB;
     double *vec = f.DinvT + nblk * kNB * kNB;
     f.scale = vec;
     double *b = vec + m, *x = b + m, *r = x + m, *z = r + m, *pd = z + m, *q = pd + m, *wk = q + m,
-           *wk2 = wk + m;
-    f.tmp = wk2 + m;  // n_data
+           *wk2 = wk + m, *pow2 = wk2 + m;
+    f.tmp = pow2 + m;  // n_data
     f.part = f.tmp + nd;
     f.dinfo = reinterpret_cast<int *>(f.part + 256);
     const unsigned gm = blocks_for(m), gy = (unsigned)(nd < 8192 ? nd : 8192);
@@ -1013,8 +1261,18 @@ int hmx_eqs_fit(const double *easting, const double *northing, const double *upw
                        (const double *)nullptr, b);
     HMX_LAUNCHED("jt_apply_kernel");
 
-    // 2. X = W^1/2 J S and the normal matrix in the mode's precision.
-    if (mixed) {
+    // 2. X = W^1/2 J S (or its slices) and the normal matrix in the mode's
+    // precision.
+    if (split) {
+        hipLaunchKernelGGL(colscale_pow2_kernel, dim3(gm), dim3(256), 0, s, easting, northing,
+                           upward, weights, nd, p_easting, p_northing, p_upward, f.scale, m, pow2);
+        HMX_LAUNCHED("colscale_pow2_kernel");
+        const unsigned gg = (unsigned)(n_groups < 4096 ? n_groups : 4096);
+        hipLaunchKernelGGL(slice_kernel, dim3(gm, gg), dim3(256), 0, s, easting, northing, upward,
+                           weights, nd, p_easting, p_northing, p_upward, f.scale, pow2, m,
+                           n_groups, static_cast<bf16x8_t *>(X));
+        HMX_LAUNCHED("slice_kernel");
+    } else if (mixed) {
         hipLaunchKernelGGL(scaled_jacobian_kernel<float>, dim3(gm, gy), dim3(256), 0, s, easting,
                            northing, upward, weights, nd, p_easting, p_northing, p_upward, f.scale,
                            m, static_cast<float *>(X), m);
@@ -1023,16 +1281,19 @@ int hmx_eqs_fit(const double *easting, const double *northing, const double *upw
                            northing, upward, weights, nd, p_easting, p_northing, p_upward, f.scale,
                            m, static_cast<double *>(X), m);
     }
-    HMX_LAUNCHED("scaled_jacobian_kernel");
+    if (!split) HMX_LAUNCHED("scaled_jacobian_kernel");
 
-    // The normal matrix of FP32 inputs (FP64 sums) differs from the FP64 one
-    // by about 2^-24 of its entries; with damping small against its largest
+    // The FP32-MFMA normal matrix differs from the FP64 one by about 2^-24
+    // of its entries (input rounding and FP32 sums within each MFMA), the
+    // split-bf16 one by about 2^-31; with damping small against the largest
     // eigenvalue that can make it indefinite.  A failed factorisation is
     // retried with a shift tau (then tenfold larger), and the factor of the
     // (shifted) low-precision matrix serves as the preconditioner of CG on
     // the exact FP64 operator.
     double tau = 0.0, dmax = 0.0;
     auto diag_max = [&](double *out) -> int {
+        if (split)  // the diagonal of the saved normal matrix
+            return f.reduce(1, A_copy, nullptr, m, m + 1, out);
         if (mixed)
             hipLaunchKernelGGL(colsumsq_kernel<float>, dim3(gm), dim3(256), 0, s,
                                static_cast<const float *>(X), m, nd, m, wk);
@@ -1050,9 +1311,21 @@ int hmx_eqs_fit(const double *easting, const double *northing, const double *upw
             hipLaunchKernelGGL(add_diag_kernel, dim3(gm), dim3(256), 0, s, f.A, m, tau);
             HMX_LAUNCHED("add_diag_kernel");
         } else if (mixed) {
-            st = launch_syrk<float>(static_cast<const float *>(X), m, nd, m, f.A, m, 0.0, 1.0,
-                                    damping, s);
-            if (st) return st;
+            if (split) {
+                const int64_t tiles = (m + kST - 1) / kST;
+                const unsigned blocks = (unsigned)(tiles * (tiles + 1) / 2);
+                const bf16x8_t *Sl = static_cast<const bf16x8_t *>(X);
+                hipLaunchKernelGGL((gram_bf16_kernel<0, 1>), dim3(blocks), dim3(512), 0, s, Sl,
+                                   n_groups, m, pow2, f.A, m, 0.0, damping, tiles);
+                HMX_LAUNCHED("gram_bf16_kernel<0,1>");
+                hipLaunchKernelGGL((gram_bf16_kernel<2, 3>), dim3(blocks), dim3(512), 0, s, Sl,
+                                   n_groups, m, pow2, f.A, m, 1.0, 0.0, tiles);
+                HMX_LAUNCHED("gram_bf16_kernel<2,3>");
+            } else {
+                st = launch_syrk<float>(static_cast<const float *>(X), m, nd, m, f.A, m, 0.0, 1.0,
+                                        damping, s);
+                if (st) return st;
+            }
             HMX_HIP(hipMemcpyAsync(A_copy, f.A, (size_t)m * m * sizeof(double),
                                    hipMemcpyDeviceToDevice, s));
         } else {
@@ -1069,10 +1342,13 @@ int hmx_eqs_fit(const double *easting, const double *northing, const double *upw
             st = diag_max(&dmax);
             if (st) return st;
         }
-        // First shift: about the rounding of the normal matrix in the mode's
-        // input precision (2^-26 for FP32 inputs with FP64 sums, 1e-14 for
-        // FP64); then tenfold.
-        tau = tau > 0.0 ? 10.0 * tau : (mixed ? std::ldexp(dmax, -26) : 1e-14 * dmax);
+        // First shift: about the error of the normal matrix in the mode, as
+        // a fraction of its largest diagonal entry (2^-22 for FP32 MFMA sums,
+        // 2^-27 for the bf16 slices, whose fixed-point columns carry 2^-32 of
+        // their largest entry; 1e-14 for FP64), then tenfold.
+        const double tau0 = split ? std::ldexp(dmax, -27) : mixed ? std::ldexp(dmax, -22)
+                                                                  : 1e-14 * dmax;
+        tau = tau > 0.0 ? 10.0 * tau : tau0;
     }
 
     // 3. x = (factor)^-1 b.

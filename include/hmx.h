@@ -61,10 +61,13 @@ extern "C" {
  * (EquivalentSources(dtype=...), cartesian.py:165-191). */
 #define HMX_F64 0
 #define HMX_F32 1
+#define HMX_BF16 2 /* hmx_mfma_probe only */
 
 /* Precision modes of hmx_eqs_fit. */
 #define HMX_FIT_FP64 0      /* J^T W J on FP64 MFMA, FP64 Cholesky, direct solve */
 #define HMX_FIT_FP32_MFMA 1 /* J^T W J on FP32 MFMA, FP64 Cholesky, FP64-residual CG */
+#define HMX_FIT_BF16_SPLIT 2 /* J^T W J from exact bf16 slice products on MFMA, FP64
+                                Cholesky, FP64-residual CG */
 
 /* Last error message of the calling thread ("" if none). */
 const char *hmx_last_error(void);
@@ -235,9 +238,13 @@ int hmx_eqs_fit_workspace_bytes(int64_t n_data, int64_t n_src, int precision_mod
  * stands for damping=None), unscaling of the coefficients.
  *   data, weights (may be NULL): (n_data) device arrays;
  *   precision_mode: HMX_FIT_FP64 (FP64 MFMA normal matrix, FP64 Cholesky,
- *     direct solve) or HMX_FIT_FP32_MFMA (FP32 Jacobian and FP32-MFMA normal
- *     matrix, shifted, factorised in FP64 and used as the preconditioner of
- *     conjugate gradients on the exact FP64 normal equations);
+ *     direct solve), HMX_FIT_FP32_MFMA (FP32 Jacobian and FP32-MFMA normal
+ *     matrix) or HMX_FIT_BF16_SPLIT (the scaled Jacobian cut into four
+ *     8-bit integer slices per column, normal matrix from the exact bf16
+ *     MFMA products of the slice pairs up to order 3, about 2^-31 of its
+ *     entries); in both low-precision modes the normal matrix, shifted if
+ *     needed, is factorised in FP64 and used as the preconditioner of
+ *     conjugate gradients on the exact FP64 normal equations;
  *   max_iter, rtol: CG limit and stop rule (relative change of the
  *     data-space prediction per iteration);
  *   coefs: (n_src) device output;
@@ -253,9 +260,11 @@ int hmx_eqs_fit(const double *easting, const double *northing, const double *upw
 
 /* Building blocks of the fit, exposed for testing and reuse (no
  * counterpart in the reference):
- * hmx_mfma_probe: one 16x16x4 MFMA (dtype HMX_F64 or HMX_F32) of a single
- *   wavefront with lane operands a[l], b[l] (64 each); out[4 l + v] is
- *   element v of lane l's accumulator (documents the operand layout).
+ * hmx_mfma_probe: one MFMA of a single wavefront; out[4 l + v] is element
+ *   v of lane l's accumulator (documents the operand layout).  HMX_F64 and
+ *   HMX_F32: 16x16x4 with lane operands a[l], b[l] (64 values each);
+ *   HMX_BF16: 16x16x32 with lane operands a[8 l .. 8 l + 7] and
+ *   b[8 l .. 8 l + 7] (512 values each, exactly representable in bf16).
  * hmx_gemm_tn: C = beta C + sign A^T B + shift I on MFMA, A (K x M, lda),
  *   B (K x N, ldb) row-major in dtype, C (M x N, ldc) FP64; upper != 0
  *   computes only the upper triangle (M == N).

@@ -224,13 +224,14 @@ def test_default_depth_fit(hm, synthetic):
     npt.assert_allclose(eqs.depth_, 4.5 * spacing)
 
 
+@pytest.mark.parametrize("precision", ["mixed", "mixed-fp32"])
 @pytest.mark.parametrize("damping", [0.1, 1e-3])
-def test_mixed_precision_fit_matches_fp64(hm, damping):
-    """FP32-MFMA preconditioned fit (FP32 inputs, FP64 sums, FP64 residual)
-    against the FP64 fit: predictions on a grid agree to 1e-6 of their
-    maximum, also at damping 1e-3 where the normal matrix has cond ~ 1e11
-    (there the FP32-rounded normal matrix needs a shift and the refinement
-    takes tens of iterations)."""
+def test_mixed_precision_fit_matches_fp64(hm, damping, precision):
+    """Preconditioned mixed-precision fits (bf16 slice products, or FP32 MFMA
+    with FP64 sums; FP64 Cholesky and FP64 residuals) against the FP64 fit:
+    predictions on a grid agree to 1e-6 of their maximum, also at damping
+    1e-3 where the normal matrix has cond ~ 1e11 (there the FP32 normal
+    matrix needs a shift and the refinement takes tens of iterations)."""
     rng = np.random.default_rng(9)
     n = 3000
     L = np.sqrt(n * 0.16e6)
@@ -239,7 +240,9 @@ def test_mixed_precision_fit_matches_fp64(hm, damping):
     data = oracle.prism_gravity(coords, prisms, [300.0], "g_z") + rng.normal(0, 0.05, n)
     grid = grid_coordinates((0, L, 0, L), shape=(40, 40), extra=1000.0)
     fp64 = hm.EquivalentSources(damping=damping, depth=4500).fit(coords, data)
-    mixed = hm.EquivalentSources(damping=damping, depth=4500, fit_precision="mixed").fit(coords, data)
+    mixed = hm.EquivalentSources(damping=damping, depth=4500, fit_precision=precision).fit(coords, data)
     p64, pmx = fp64.predict(grid), mixed.predict(grid)
     err = np.abs(pmx - p64).max() / np.abs(p64).max()
     assert err <= 1e-6, (err, mixed.fit_info_)
+    if precision == "mixed":  # the split normal matrix is close to the FP64 one
+        assert mixed.fit_info_["iterations"] <= 30, mixed.fit_info_

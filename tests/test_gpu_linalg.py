@@ -58,6 +58,37 @@ def test_mfma_layout(env, dtype):
     assert np.array_equal(got, expected), f"matching layouts: {_which(a, b, got)}"
 
 
+def test_mfma_layout_bf16(env):
+    """16x16x32 bf16 MFMA: lane l supplies row l % 16 of A and column l % 16
+    of B, 8 values each of the k block l / 16; element v of its result is
+    D[4 (l / 16) + v][l % 16].  The order of k inside the 32 is invisible
+    in the product as long as A and B use the same one (the Gram kernel
+    does), so both orders are accepted."""
+    torch, _lib, L = env
+    rng = np.random.default_rng(1)
+    a = rng.integers(-8, 9, 512).astype(float)
+    b = rng.integers(-8, 9, 512).astype(float)
+    A, B = torch.tensor(a).cuda(), torch.tensor(b).cuda()
+    out = torch.empty(256, dtype=torch.float64, device="cuda")
+    _lib.check(L.hmx_mfma_probe(2, _lib.ptr(A), _lib.ptr(B), _lib.ptr(out), None))
+    got = out.cpu().numpy()
+    layouts = {}
+    for kname, kmap in (("k=8(l/16)+e", lambda l, e: 8 * (l // 16) + e),
+                        ("k=(l/16)+4e", lambda l, e: l // 16 + 4 * e)):
+        Am, Bm = np.zeros((16, 32)), np.zeros((32, 16))
+        for l in range(64):
+            for e in range(8):
+                Am[l % 16, kmap(l, e)] = a[8 * l + e]
+                Bm[kmap(l, e), l % 16] = b[8 * l + e]
+        D = Am @ Bm
+        for dname, dmap in (("D[4(l/16)+v][l%16]", lambda l, v: (4 * (l // 16) + v, l % 16)),
+                            ("D[(l/16)+4v][l%16]", lambda l, v: (l // 16 + 4 * v, l % 16))):
+            layouts[(kname, dname)] = np.array([D[dmap(l, v)] for l in range(64) for v in range(4)])
+    matches = [k for k, v in layouts.items() if np.array_equal(v, got)]
+    assert ("k=8(l/16)+e", "D[4(l/16)+v][l%16]") in matches, matches
+    assert all(d == "D[4(l/16)+v][l%16]" for _, d in matches), matches
+
+
 def _which(a, b, got):
     """Diagnose: which (A map, B map, D map) combinations reproduce `got`."""
     amaps = {"A[l%16][l/16]": lambda l: (l % 16, l // 16), "A[l/4][l%4]": lambda l: (l // 4, l % 4)}
