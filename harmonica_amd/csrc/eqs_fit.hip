@@ -1342,11 +1342,12 @@ int hmx_eqs_fit(const double *easting, const double *northing, const double *upw
             st = diag_max(&dmax);
             if (st) return st;
         }
-        // First shift: about the error of the normal matrix in the mode, as
-        // a fraction of its largest diagonal entry (2^-22 for FP32 MFMA sums,
-        // 2^-27 for the bf16 slices, whose fixed-point columns carry 2^-32 of
-        // their largest entry; 1e-14 for FP64), then tenfold.
-        const double tau0 = split ? std::ldexp(dmax, -27) : mixed ? std::ldexp(dmax, -22)
+        // First shift, as a fraction of the largest diagonal entry: 2^-26
+        // for FP32 MFMA sums, 2^-30 for the bf16 slices (fixed-point
+        // columns carry 2^-32 of their largest entry), 1e-14 for FP64; then
+        // tenfold.  A shift just above the error of the normal matrix keeps
+        // the conjugate-gradient refinement short.
+        const double tau0 = split ? std::ldexp(dmax, -30) : mixed ? std::ldexp(dmax, -26)
                                                                   : 1e-14 * dmax;
         tau = tau > 0.0 ? 10.0 * tau : tau0;
     }
