@@ -188,16 +188,24 @@ def predict_device(coords, points, coefs, dtype=torch.float64):
 
 
 def fit_device(coords, data, weights, points, damping, precision="float64", max_iter=None,
-               rtol=1e-9):
+               rtol=None):
     """Coefficients of the scaled, damped, weighted least-squares fit on the
-    GPU (all inputs float64).  Returns (coefs numpy array, info dict)."""
+    GPU (all inputs float64).  Returns (coefs numpy array, info dict).
+
+    Every mode refines its solution by conjugate gradients on the FP64
+    normal equations, preconditioned by its Cholesky factor: the FP64 mode
+    for a few iterations (at cond ~ 1e11 the direct solve alone is off by
+    ~1e-6 in the predictions), the mixed modes until the relative update of
+    the predictions falls below rtol."""
     import ctypes
 
     modes = {"float64": _lib.HMX_FIT_FP64, "mixed": _lib.HMX_FIT_BF16_SPLIT,
              "mixed-fp32": _lib.HMX_FIT_FP32_MFMA}
     mode = modes[precision]
     if max_iter is None:
-        max_iter = 0 if mode == _lib.HMX_FIT_FP64 else 300
+        max_iter = 10 if mode == _lib.HMX_FIT_FP64 else 300
+    if rtol is None:
+        rtol = 1e-12 if mode == _lib.HMX_FIT_FP64 else 1e-9
     c = [_dev(x, torch.float64) for x in coords]
     p = [_dev(x, torch.float64) for x in points]
     d = _dev(data, torch.float64)
