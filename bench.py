@@ -14,7 +14,7 @@ The second half of the metric, the equivalent-source fit of 60,000 data
 headline steps and reported under "extra".
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-eqs]
-       [--eqs-all] [--eqs-oracle] [--extra] [--layer] [--batched] [--inversion]
+       [--eqs-all] [--eqs-oracle] [--extra] [--layer] [--point] [--batched] [--inversion]
 """
 
 import argparse
@@ -126,6 +126,32 @@ def bench_layer_fused(args):
     dt = time.perf_counter() - t0
     out["api_prism_gravity_host_arrays"] = {"s": dt, "pair_evals_per_s": pairs / dt}
     log(f"prism_gravity API from host arrays: {dt:.3f} s, {pairs / dt:.4e} pair-evals/s")
+    return out
+
+
+def bench_point(args):
+    """point_gravity on 1,000,000 point masses (the terrain prisms' centres
+    of mass and masses) at the 256 x 256 grid at 2500 m: g_z and the ten
+    fields fused (hmx_point_gravity, inputs in HBM)."""
+    from harmonica_amd._forward import point_fields_device, to_device
+    from harmonica_amd._lib import FIELDS
+    from harmonica_amd.workloads import layer_to_prisms, terrain_layer_large
+
+    w = terrain_layer_large(n_prisms_side=args.prism_side, n_obs_side=args.obs_side)
+    prisms = layer_to_prisms(w["easting"], w["northing"], w["surface"], w["reference"])
+    centres = [0.5 * (prisms[:, 2 * a] + prisms[:, 2 * a + 1]) for a in range(3)]
+    vol = np.prod([prisms[:, 2 * a + 1] - prisms[:, 2 * a] for a in range(3)], axis=0)
+    masses = w["density"].ravel() * vol
+    E, N, U = (to_device(c.ravel()) for c in w["coordinates"])
+    P = [to_device(c) for c in centres]
+    M = to_device(masses)
+    pairs = E.numel() * M.numel()
+    out = {}
+    for name, fields in (("g_z", ("g_z",)), ("all_ten", FIELDS)):
+        point_fields_device(E, N, U, *P, M, fields)
+        _, dt = _sync_time(lambda: point_fields_device(E, N, U, *P, M, fields))
+        out[name] = {"s": dt, "pair_evals_per_s": pairs / dt}
+        log(f"point {name}: {dt:.4f} s, {pairs / dt:.4e} pair-evals/s")
     return out
 
 
@@ -261,6 +287,7 @@ def main():
     ap.add_argument("--obs-side", type=int, default=256)
     ap.add_argument("--prism-side", type=int, default=1000)
     ap.add_argument("--layer", action="store_true", help="also time the layer accessor path")
+    ap.add_argument("--point", action="store_true", help="also time point_gravity (1M masses)")
     ap.add_argument("--batched", action="store_true", help="also run the 512-model batch")
     ap.add_argument("--batched-models", type=int, default=512)
     ap.add_argument("--no-eqs", action="store_true",
@@ -361,6 +388,8 @@ def main():
 
     if args.layer:
         extra["layer_path"] = bench_layer_fused(args)
+    if args.point:
+        extra["point_masses"] = bench_point(args)
     if args.batched:
         extra["batched"] = bench_batched(args)
     if not args.no_eqs:

@@ -21,6 +21,7 @@ __global__ __launch_bounds__(256) void devmath_eval_kernel(int fn, const double 
         case HMX_DM_LOG: r = dm::log_pos(a[i], tables); break;
         case HMX_DM_ATAN_RATIO: r = dm::atan_ratio(a[i], b[i], tables); break;
         case HMX_DM_ATAN2: r = dm::atan2_full(a[i], b[i], tables); break;
+        case HMX_DM_RSQRT: r = dm::rsqrt_pos(a[i]); break;
     }
     out[i] = r;
 }
@@ -32,7 +33,7 @@ extern "C" int hmx_devmath_eval(int fn, const double *a, const double *b, double
                                 void *stream) {
     using namespace hmx;
     clear_error();
-    if (fn < HMX_DM_SQRT || fn > HMX_DM_ATAN2) return fail(HMX_ERR_ARG, "unknown function %d", fn);
+    if (fn < HMX_DM_SQRT || fn > HMX_DM_RSQRT) return fail(HMX_ERR_ARG, "unknown function %d", fn);
     if (n < 0) return fail(HMX_ERR_ARG, "negative size");
     if (n == 0) return HMX_OK;
     if (!a || !out || ((fn == HMX_DM_DIV || fn == HMX_DM_ATAN_RATIO || fn == HMX_DM_ATAN2) && !b))

@@ -5,10 +5,16 @@
 // stores; the kernel is HBM-write bound).
 // predict: 256 observation points per workgroup, sources and coefficients
 // staged through LDS, FP64 accumulation (also for FP32 data).
+#include "devmath.h"
 #include "hmx_internal.h"
 
 namespace hmx {
 namespace {
+
+// 1 / sqrt(s): FP64 through the library's reciprocal square root (about one
+// ulp), FP32 in single precision (the reference's float32 jacobian).
+__device__ __forceinline__ double inv_sqrt(double s) { return dm::rsqrt_pos(s); }
+__device__ __forceinline__ float inv_sqrt(float s) { return 1.0f / sqrtf(s); }
 
 template <class T>
 __global__ __launch_bounds__(256) void eqs_jacobian_kernel(
@@ -20,7 +26,7 @@ __global__ __launch_bounds__(256) void eqs_jacobian_kernel(
     const T sj = pe[j], nj = pn[j], uj = pu[j];
     for (int64_t i = blockIdx.y; i < n_data; i += gridDim.y) {
         const T de = e[i] - sj, dn = n[i] - nj, du = u[i] - uj;
-        jac[i * ld + j] = T(1) / sqrt(de * de + dn * dn + du * du);
+        jac[i * ld + j] = inv_sqrt(de * de + dn * dn + du * du);
     }
 }
 
@@ -57,7 +63,7 @@ __global__ __launch_bounds__(kObsPerBlock) void eqs_predict_kernel(
         for (int p = 0; p < cnt; ++p) {
             const StagedSource<T> s = tile[p];
             const T de = oe - s.e, dn = on - s.n, du = ou - s.u;
-            acc += (double)(s.c * (T(1) / sqrt(de * de + dn * dn + du * du)));
+            acc += (double)(s.c * inv_sqrt(de * de + dn * dn + du * du));
         }
     }
     if (valid) partial[(int64_t)blockIdx.y * n_obs + i] = acc;

@@ -42,7 +42,7 @@ constexpr int kPad = 144;   // LDS row stride (doubles/floats) of the SYRK panel
 __device__ __forceinline__ double inv_dist(double ae, double an, double au, double be, double bn,
                                            double bu) {
     const double de = ae - be, dn = an - bn, du = au - bu;
-    return dm::div(1.0, dm::sqrt_nn(de * de + dn * dn + du * du));
+    return dm::rsqrt_pos(de * de + dn * dn + du * du);
 }
 
 // ------------------------------------------------------------ column stats

@@ -22,8 +22,7 @@ __device__ __forceinline__ void point_pair(double e, double n, double u, double 
     using F = FieldSet<M>;
     const double de = e - pe, dn = n - pn, du = u - pu;
     const double d2 = de * de + dn * dn + du * du;
-    const double d = sqrt(d2);
-    const double inv = 1.0 / d;
+    const double inv = dm::rsqrt_pos(d2);
     const double inv3 = inv * inv * inv;
     const double inv5 = inv3 * inv * inv;
     if constexpr (F::pot) acc[F::slot(0)] += gm * inv;

@@ -288,6 +288,7 @@ int hmx_cholesky_solve(double *A, int64_t m, double *x, int *info, void *workspa
 #define HMX_DM_LOG 2        /* ln(a), a > 0 */
 #define HMX_DM_ATAN_RATIO 3 /* atan(a / b), +-pi/2 when b == 0 */
 #define HMX_DM_ATAN2 4      /* atan2(a, b) in (-pi, pi], a and b not both 0 */
+#define HMX_DM_RSQRT 5      /* 1 / sqrt(a), a > 0 */
 int hmx_devmath_eval(int fn, const double *a, const double *b, double *out, int64_t n,
                      void *stream);
 

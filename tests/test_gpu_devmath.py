@@ -8,7 +8,8 @@ Bounds (u = ulp of the exact result):
   fma, so near log = 0 the error is absolute, which is what the vertex sums
   need: each log is multiplied by a coordinate and summed);
 * atan of a ratio: <= 1.5 u + 2^-54 absolute; exact +-pi/2 and 0 in the
-  degenerate cases.
+  degenerate cases;
+* reciprocal square root: <= 1.5 u.
 """
 
 import ctypes
@@ -20,7 +21,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 mpmath.mp.dps = 50
 
-SQRT, DIV, LOG, ATAN, ATAN2 = 0, 1, 2, 3, 4
+SQRT, DIV, LOG, ATAN, ATAN2, RSQRT = 0, 1, 2, 3, 4, 5
 
 
 @pytest.fixture(scope="module")
@@ -133,3 +134,12 @@ def test_atan2(ev):
                  np.array([1.0, -1.0, 0.0, 0.0, -2.0, -2.0]))
     np.testing.assert_allclose(special, [0.0, np.pi, np.pi / 2, -np.pi / 2, 0.75 * np.pi,
                                          -0.75 * np.pi], rtol=1e-15, atol=0)
+
+
+def test_rsqrt(ev):
+    rng = np.random.default_rng(5)
+    a = np.concatenate([_wide(rng, 3000, 1e-20, 1e20), np.arange(1, 200, dtype=float) ** 2,
+                        2.0 ** np.arange(-60, 60)])
+    got = ev(RSQRT, a)
+    worst = max(ulp_err(g, 1 / mpmath.sqrt(mpmath.mpf(x))) for g, x in zip(got, a))
+    assert worst <= 1.5, worst
