@@ -36,23 +36,23 @@ FP64_PEAK = 78.6e12
 # (prism_forward_kernel<g_z>, grouped closed form): rocprofv3 counters
 # SQ_INSTS_VALU_{ADD,MUL,TRANS}_F64 + 2 x SQ_INSTS_VALU_FMA_F64, x 64 lanes,
 # divided by the pairs of the profiled launch
-# (profiles/prism_gz_terrain1M_counters.md).  The same profile gives 478.6
+# (profiles/prism_gz_terrain1M_counters.md).  The same profile gives 438.6
 # VALU instructions per pair; every wave64 VALU instruction occupies a SIMD
 # for 4 cycles, which bounds the pair rate at 1024 SIMDs x 2.4 GHz x 16 /
-# 478.6.  The per-vertex form of the same closed form (8 square roots, 16
+# 438.6.  The per-vertex form of the same closed form (8 square roots, 16
 # logarithms, 8 arctangents per pair with this library's routines) measured
 # 1083.7 FP64 operations per pair; it is reported beside the executed count.
-FLOPS_PER_PAIR_GZ = 505.0
-VALU_INSTR_PER_PAIR_GZ = 478.6
+FLOPS_PER_PAIR_GZ = 465.0
+VALU_INSTR_PER_PAIR_GZ = 438.6
 PER_VERTEX_FLOPS_PER_PAIR_GZ = 1083.7
 # Same counters for the ten fields fused (prism_forward_kernel<all>).
-FLOPS_PER_PAIR_ALL = 1106.0
-VALU_INSTR_PER_PAIR_ALL = 1238.8
+FLOPS_PER_PAIR_ALL = 1066.0
+VALU_INSTR_PER_PAIR_ALL = 1198.8
 VALU_ISSUE_RATE = 1024 * 2.4e9 * 16  # lane-instructions per second, wave64 at 4 cycles
 # HBM bytes per launch of the g_z kernel on this workload: FETCH_SIZE
-# 219591 KiB x 2 (a 1 GiB copy reads as 524308 KiB) + WRITE_SIZE 20480 KiB
+# 219588.7 KiB x 2 (a 1 GiB copy reads as 524308 KiB) + WRITE_SIZE 20480 KiB
 # x 1 (profiles/prism_gz_terrain1M_counters.md).
-TRAFFIC_BYTES_GZ = (219591.0 * 2 + 20480.0) * 1024
+TRAFFIC_BYTES_GZ = (219588.7 * 2 + 20480.0) * 1024
 
 
 def log(msg):

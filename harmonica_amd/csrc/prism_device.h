@@ -40,6 +40,13 @@ namespace dev {
 
 constexpr double kG = 6.6743e-11;  // harmonica/constants.py:12
 
+// Vertex distance r = sqrt(x^2 + y^2 + z^2).  2^-1000 keeps the argument
+// positive (sqrt_pos needs s > 0) and changes nothing above 1e-284 m^2; r
+// is then ~1e-150 instead of 0 only when the point is the vertex, where
+// every grouped term that uses it has a zero coefficient or takes the
+// face-plane branch.
+__device__ __forceinline__ double vertex_r(double s) { return dm::sqrt_pos(s + 0x1p-1000); }
+
 // Compile-time description of a set of fields (bit f = FIELDS[f]).
 template <unsigned M>
 struct FieldSet {
@@ -158,7 +165,7 @@ __device__ __forceinline__ double gu_sum_paired(double x0, double x1, double y0,
         for (int j = 0; j < 2; ++j)
 #pragma unroll
             for (int k = 0; k < 2; ++k)
-                r[i][j][k] = dm::sqrt_nn((i ? xs1 : xs0) + (j ? ys1 : ys0) + (k ? zs1 : zs0));
+                r[i][j][k] = vertex_r((i ? xs1 : xs0) + (j ? ys1 : ys0) + (k ? zs1 : zs0));
     double sum = 0.0, top = 0.0;
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
@@ -230,7 +237,7 @@ __device__ __forceinline__ void prism_pair(double e, double n, double u, double 
 #pragma unroll
         for (int j = 0; j < 2; ++j)
 #pragma unroll
-            for (int k = 0; k < 2; ++k) r[i][j][k] = dm::sqrt_nn(xs[i] + ys[j] + zs[k]);
+            for (int k = 0; k < 2; ++k) r[i][j][k] = vertex_r(xs[i] + ys[j] + zs[k]);
     // Vertex sums grouped by the coordinates each term depends on:
     //   sum (-1)^(i+j+k) f(y_j, z_k) L_x = sum_{j,k} (-1)^(j+k) f(y_j, z_k) DLx[j][k]
     //   with DLx[j][k] = L_x(0,j,k) - L_x(1,j,k) (one logarithm), and likewise
