@@ -250,8 +250,8 @@ def bench_inversion(args):
     pairs = geom.n_obs * geom.n_prisms
     _, t_fwd = _sync_time(lambda: geom.forward(rho, top))
     v = torch.ones_like(data)
+    g_rho, g_top = geom.vjp(rho, top, v)  # first call also warms the backward kernels
     _, t_bwd = _sync_time(lambda: geom.vjp(rho, top, v))
-    g_rho, g_top = geom.vjp(rho, top, v)
     # central differences on three prisms, FP64, steps 1e-4 kg/m^3 and 1e-3 m
     pts = [c.ravel() for c in w["coordinates"]]
     he = (w["easting"][1] - w["easting"][0]) / 2

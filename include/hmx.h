@@ -64,7 +64,8 @@ extern "C" {
 #define HMX_BF16 2 /* hmx_mfma_probe only */
 
 /* Precision modes of hmx_eqs_fit. */
-#define HMX_FIT_FP64 0      /* J^T W J on FP64 MFMA, FP64 Cholesky, direct solve */
+#define HMX_FIT_FP64 0      /* J^T W J on FP64 MFMA, FP64 Cholesky, direct solve refined
+                               by max_iter CG iterations (0: direct solve only) */
 #define HMX_FIT_FP32_MFMA 1 /* J^T W J on FP32 MFMA, FP64 Cholesky, FP64-residual CG */
 #define HMX_FIT_BF16_SPLIT 2 /* J^T W J from exact bf16 slice products on MFMA, FP64
                                 Cholesky, FP64-residual CG */
