@@ -40,12 +40,14 @@ namespace dev {
 
 constexpr double kG = 6.6743e-11;  // harmonica/constants.py:12
 
-// Vertex distance r = sqrt(x^2 + y^2 + z^2).  2^-1000 keeps the argument
+// Vertex distance r = sqrt((x^2 + y^2) + zt) with zt = z^2 + 2^-1000
+// (z_tiny below, formed once per face).  The 2^-1000 keeps the argument
 // positive (sqrt_pos needs s > 0) and changes nothing above 1e-284 m^2; r
 // is then ~1e-150 instead of 0 only when the point is the vertex, where
 // every grouped term that uses it has a zero coefficient or takes the
 // face-plane branch.
-__device__ __forceinline__ double vertex_r(double s) { return dm::sqrt_pos(s + 0x1p-1000); }
+__device__ __forceinline__ double z_tiny(double zs) { return zs + 0x1p-1000; }
+__device__ __forceinline__ double vertex_r(double xys, double zt) { return dm::sqrt_pos(xys + zt); }
 
 // Compile-time description of a set of fields (bit f = FIELDS[f]).
 template <unsigned M>
@@ -73,17 +75,23 @@ struct FieldSet {
 //   a_1 >= 0:         ln(t00 t11 / (t10 t01))
 //   a_0 < 0:         -ln(t00 t11 / (t10 t01))
 //   a_0 >= 0 > a_1:   ln(t00 t10 w2_1 / (t01 t11 w2_0)).
-// An argument can vanish only where the logarithm's coefficient is zero;
-// the caller drops those terms.
+// The caller passes w2 + 2^-1000 (z_tiny): every t is at least 2^-500 and
+// w2 at least 2^-1000, so for a prism of nonzero extent the quotient is a
+// positive normal number and the logarithm finite even where the true
+// argument vanishes -- which happens only where the logarithm's
+// coefficient is zero, so that term is exactly 0 without a test.
 __device__ __forceinline__ double log_quad(double a0, double a1, double r00, double r01,
                                            double r10, double r11, double w20, double w21,
                                            const dm::Tables &tab) {
-    const bool pos1 = a1 >= 0.0, neg0 = a0 < 0.0, mixed = !pos1 && !neg0;
+    const bool neg0 = a0 < 0.0, mixed = !neg0 && a1 < 0.0;
     const double t00 = r00 + fabs(a0), t01 = r01 + fabs(a0);
     const double t10 = r10 + fabs(a1), t11 = r11 + fabs(a1);
-    const double num = mixed ? t00 * t10 * w21 : t00 * t11;
-    const double den = mixed ? t01 * t11 * w20 : t10 * t01;
-    const double l = dm::log_pos(num * dm::rcp(den), tab);
+    double num = t00 * t11, den = t10 * t01;
+    if (__builtin_expect(mixed, 0)) {  // the point lies between the two a-planes
+        num = t00 * (t10 * w21);
+        den = t01 * (t11 * w20);
+    }
+    const double l = dm::log_pos(dm::div(num, den), tab);
     return neg0 ? -l : l;
 }
 
@@ -96,7 +104,8 @@ __device__ __forceinline__ double log_quad(double a0, double a1, double r00, dou
 // (N_0 D_1 - N_1 D_0, D_0 D_1 + N_0 N_1), correct modulo 2 pi.  The solid
 // angle lies in (-pi, pi) unless the point projects strictly inside the
 // rectangle, where it has the sign of z and magnitude up to 2 pi; that case
-// takes the branch with the sign of z.
+// takes the branch with the sign of z.  For z = 0 the result is 0 (the
+// angle pair is (0, D >= 0)).
 __device__ __forceinline__ double face_angle(double x0, double x1, double y0, double y1, double z,
                                              double r00, double r01, double r10, double r11,
                                              bool inside, const dm::Tables &tab) {
@@ -106,8 +115,8 @@ __device__ __forceinline__ double face_angle(double x0, double x1, double y0, do
     const double d1 = fma(z * z, r01 * r11, x0 * x1 * (y1 * y1));
     const double a = dm::atan2_full(fma(n0, d1, -n1 * d0), fma(d0, d1, n0 * n1), tab);
     constexpr double kTwoPi = 6.283185307179586232;
-    const bool wrap = inside && (z > 0.0 ? a < 0.0 : a > 0.0);
-    return wrap ? (z > 0.0 ? a + kTwoPi : a - kTwoPi) : a;
+    const bool wrap = inside && a * z < 0.0;
+    return wrap ? a + copysign(kTwoPi, z) : a;
 }
 
 // ln(P_0 / P_1) for two safe_log arguments that differ only in the
@@ -125,7 +134,7 @@ __device__ __forceinline__ double log_diff(double a0, double a1, double r0, doub
     const bool mixed = a0 >= 0.0 && a1 < 0.0;
     const double num = line ? (a0 > 0.0 ? t0 : 1.0) : (mixed ? t0 * t1 : t0);
     const double den = line ? (a1 > 0.0 ? t1 : 1.0) : (mixed ? w2 : t1);
-    const double l = dm::log_pos(num * dm::rcp(den), tab);
+    const double l = dm::log_pos(dm::div(num, den), tab);
     return (a0 < 0.0 && !line) ? -l : l;
 }
 
@@ -150,7 +159,8 @@ __device__ __forceinline__ double face_sum(double a0, double a1, double b0, doub
 // grouped terms are formed without the cancellation between vertex terms
 // of far prisms, so they are also more accurate.
 // Terms whose coefficient (x_i, y_j or z_k) is zero are zero, as in the
-// per-vertex form.  With kTop, *ftop receives the top-face sum
+// per-vertex form (their logarithms and angles are finite, see log_quad and
+// face_angle).  With kTop, *ftop receives the top-face sum
 // sum_{i,j} (-1)^(i+j) A_z(i, j, 0) (the derivative with respect to the top).
 template <bool kTop>
 __device__ __forceinline__ double gu_sum_paired(double x0, double x1, double y0, double y1,
@@ -158,6 +168,7 @@ __device__ __forceinline__ double gu_sum_paired(double x0, double x1, double y0,
                                                 double *ftop) {
     const double xs0 = x0 * x0, xs1 = x1 * x1, ys0 = y0 * y0, ys1 = y1 * y1;
     const double zs0 = z0 * z0, zs1 = z1 * z1;
+    const double zt0 = z_tiny(zs0), zt1 = z_tiny(zs1);
     double r[2][2][2];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
@@ -165,21 +176,21 @@ __device__ __forceinline__ double gu_sum_paired(double x0, double x1, double y0,
         for (int j = 0; j < 2; ++j)
 #pragma unroll
             for (int k = 0; k < 2; ++k)
-                r[i][j][k] = vertex_r((i ? xs1 : xs0) + (j ? ys1 : ys0) + (k ? zs1 : zs0));
+                r[i][j][k] = vertex_r((i ? xs1 : xs0) + (j ? ys1 : ys0), k ? zt1 : zt0);
     double sum = 0.0, top = 0.0;
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
         const double c = i ? x1 : x0, cs = i ? xs1 : xs0;
         const double l = log_quad(y0, y1, r[i][0][0], r[i][0][1], r[i][1][0], r[i][1][1],
-                                  cs + zs0, cs + zs1, tab);
-        sum = c == 0.0 ? sum : fma(i ? -c : c, l, sum);
+                                  cs + zt0, cs + zt1, tab);
+        sum = fma(i ? -c : c, l, sum);
     }
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
         const double c = j ? y1 : y0, cs = j ? ys1 : ys0;
         const double l = log_quad(x0, x1, r[0][j][0], r[0][j][1], r[1][j][0], r[1][j][1],
-                                  cs + zs0, cs + zs1, tab);
-        sum = c == 0.0 ? sum : fma(j ? -c : c, l, sum);
+                                  cs + zt0, cs + zt1, tab);
+        sum = fma(j ? -c : c, l, sum);
     }
     const bool inside = x0 > 0.0 && x1 < 0.0 && y0 > 0.0 && y1 < 0.0;
 #pragma unroll
@@ -187,7 +198,7 @@ __device__ __forceinline__ double gu_sum_paired(double x0, double x1, double y0,
         const double z = k ? z1 : z0;
         const double w = face_angle(x0, x1, y0, y1, z, r[0][0][k], r[0][1][k], r[1][0][k],
                                     r[1][1][k], inside, tab);
-        sum = z == 0.0 ? sum : fma(k ? z : -z, w, sum);
+        sum = fma(k ? z : -z, w, sum);  // w = 0 for z = 0
         if constexpr (kTop) {
             if (k == 0) {
                 // z_T = 0: safe_atan(x y, 0) = +-pi/2 by the sign of x y, or 0.
@@ -237,7 +248,7 @@ __device__ __forceinline__ void prism_pair(double e, double n, double u, double 
 #pragma unroll
         for (int j = 0; j < 2; ++j)
 #pragma unroll
-            for (int k = 0; k < 2; ++k) r[i][j][k] = vertex_r(xs[i] + ys[j] + zs[k]);
+            for (int k = 0; k < 2; ++k) r[i][j][k] = vertex_r(xs[i] + ys[j], z_tiny(zs[k]));
     // Vertex sums grouped by the coordinates each term depends on:
     //   sum (-1)^(i+j+k) f(y_j, z_k) L_x = sum_{j,k} (-1)^(j+k) f(y_j, z_k) DLx[j][k]
     //   with DLx[j][k] = L_x(0,j,k) - L_x(1,j,k) (one logarithm), and likewise
