@@ -36,23 +36,35 @@ FP64_PEAK = 78.6e12
 # (prism_forward_kernel<g_z>, grouped closed form): rocprofv3 counters
 # SQ_INSTS_VALU_{ADD,MUL,TRANS}_F64 + 2 x SQ_INSTS_VALU_FMA_F64, x 64 lanes,
 # divided by the pairs of the profiled launch
-# (profiles/prism_gz_terrain1M_counters.md).  The same profile gives 438.6
-# VALU instructions per pair; every wave64 VALU instruction occupies a SIMD
-# for 4 cycles, which bounds the pair rate at 1024 SIMDs x 2.4 GHz x 16 /
-# 438.6.  The per-vertex form of the same closed form (8 square roots, 16
-# logarithms, 8 arctangents per pair with this library's routines) measured
-# 1083.7 FP64 operations per pair; it is reported beside the executed count.
-FLOPS_PER_PAIR_GZ = 465.0
-VALU_INSTR_PER_PAIR_GZ = 438.6
+# (profiles/prism_gz_terrain1M_counters.md).  The same profile gives 347.8
+# VALU instructions per pair, 16 of them FP64 transcendentals (v_rsq_f64,
+# v_rcp_f64).  A wave64 VALU instruction occupies a SIMD for 4 cycles and
+# an FP64 transcendental for 16 (measured on the MI355X), which bounds the
+# pair rate at 1024 SIMDs x 2.4 GHz x 64 lanes / (4 (VALU - TRANS) + 16
+# TRANS) cycles.  The per-vertex form of the same closed form (8 square
+# roots, 16 logarithms, 8 arctangents per pair) measured 1083.7 FP64
+# operations per pair; it is reported beside the executed count.
+FLOPS_PER_PAIR_GZ = 394.6
+VALU_INSTR_PER_PAIR_GZ = 347.8
+TRANS_PER_PAIR_GZ = 16.0
 PER_VERTEX_FLOPS_PER_PAIR_GZ = 1083.7
 # Same counters for the ten fields fused (prism_forward_kernel<all>).
-FLOPS_PER_PAIR_ALL = 1066.0
-VALU_INSTR_PER_PAIR_ALL = 1198.8
-VALU_ISSUE_RATE = 1024 * 2.4e9 * 16  # lane-instructions per second, wave64 at 4 cycles
+FLOPS_PER_PAIR_ALL = 996.0
+VALU_INSTR_PER_PAIR_ALL = 1124.8
+TRANS_PER_PAIR_ALL = 32.0
+SIMD_CYCLES_PER_S = 1024 * 2.4e9
+
+
+def valu_issue_frac(pair_rate, valu, trans):
+    """Fraction of the SIMDs' issue cycles the pair rate needs."""
+    cycles_per_pair_wave = 4.0 * (valu - trans) + 16.0 * trans
+    return pair_rate * cycles_per_pair_wave / (64.0 * SIMD_CYCLES_PER_S)
+
+
 # HBM bytes per launch of the g_z kernel on this workload: FETCH_SIZE
-# 219588.7 KiB x 2 (a 1 GiB copy reads as 524308 KiB) + WRITE_SIZE 20480 KiB
+# 219594 KiB x 2 (a 1 GiB copy reads as 524308 KiB) + WRITE_SIZE 20480 KiB
 # x 1 (profiles/prism_gz_terrain1M_counters.md).
-TRAFFIC_BYTES_GZ = (219588.7 * 2 + 20480.0) * 1024
+TRAFFIC_BYTES_GZ = (219594.0 * 2 + 20480.0) * 1024
 
 
 def log(msg):
@@ -371,7 +383,7 @@ def main():
                 "unit": "TFLOP/s", "frac": achieved / FP64_PEAK,
                 "traffic": TRAFFIC_BYTES_GZ if full_size else None,
                 "flops_per_pair": FLOPS_PER_PAIR_GZ,
-                "valu_issue_frac": value * VALU_INSTR_PER_PAIR_GZ / VALU_ISSUE_RATE,
+                "valu_issue_frac": valu_issue_frac(value, VALU_INSTR_PER_PAIR_GZ, TRANS_PER_PAIR_GZ),
                 "per_vertex_equivalent_tflops": value * PER_VERTEX_FLOPS_PER_PAIR_GZ / 1e12}
 
     extra = {}
@@ -383,7 +395,8 @@ def main():
         extra["all_ten_fields"] = {"ms_per_step": ms10, "pair_evals_per_s": rate10,
                                    "fp64_tflops": rate10 * FLOPS_PER_PAIR_ALL / 1e12,
                                    "fp64_peak_frac": rate10 * FLOPS_PER_PAIR_ALL / FP64_PEAK,
-                                   "valu_issue_frac": rate10 * VALU_INSTR_PER_PAIR_ALL / VALU_ISSUE_RATE}
+                                   "valu_issue_frac": valu_issue_frac(rate10, VALU_INSTR_PER_PAIR_ALL,
+                                                                      TRANS_PER_PAIR_ALL)}
         log(f"ten fields fused: {ms10:.2f} ms/step")
 
     if args.layer:

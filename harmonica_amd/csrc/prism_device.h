@@ -69,27 +69,27 @@ struct FieldSet {
 
 // sum_k (-1)^k ln(P(0,k) / P(1,k)) for four safe_log arguments
 // P(j,k) = a_j + r_jk if a_j >= 0, else w2_k / (r_jk - a_j), where a is the
-// logarithm's own coordinate (a_0 >= a_1) and w2_k the sum of squares of the
-// other two.  With t_jk = r_jk + |a_j| (no cancellation), P = t if a >= 0
+// logarithm's own coordinate (a_0 >= a_1) and w2_k = cs + z_k^2 the sum of
+// squares of the other two (passed as cs and zt_k = z_k^2 + 2^-1000).  With t_jk = r_jk + |a_j| (no cancellation), P = t if a >= 0
 // and w2 / t if a < 0, so the four logarithms reduce to one:
 //   a_1 >= 0:         ln(t00 t11 / (t10 t01))
 //   a_0 < 0:         -ln(t00 t11 / (t10 t01))
 //   a_0 >= 0 > a_1:   ln(t00 t10 w2_1 / (t01 t11 w2_0)).
-// The caller passes w2 + 2^-1000 (z_tiny): every t is at least 2^-500 and
-// w2 at least 2^-1000, so for a prism of nonzero extent the quotient is a
-// positive normal number and the logarithm finite even where the true
-// argument vanishes -- which happens only where the logarithm's
-// coefficient is zero, so that term is exactly 0 without a test.
+// Every t is at least 2^-500 and w2 (with its 2^-1000) at least 2^-1000,
+// so for a prism of nonzero extent the quotient is a positive normal number
+// and the logarithm finite even where the true argument vanishes -- which
+// happens only where the logarithm's coefficient is zero, so that term is
+// exactly 0 without a test.
 __device__ __forceinline__ double log_quad(double a0, double a1, double r00, double r01,
-                                           double r10, double r11, double w20, double w21,
-                                           const dm::Tables &tab) {
+                                           double r10, double r11, double cs, double zt0,
+                                           double zt1, const dm::Tables &tab) {
     const bool neg0 = a0 < 0.0, mixed = !neg0 && a1 < 0.0;
     const double t00 = r00 + fabs(a0), t01 = r01 + fabs(a0);
     const double t10 = r10 + fabs(a1), t11 = r11 + fabs(a1);
     double num = t00 * t11, den = t10 * t01;
     if (__builtin_expect(mixed, 0)) {  // the point lies between the two a-planes
-        num = t00 * (t10 * w21);
-        den = t01 * (t11 * w20);
+        num = t00 * (t10 * (cs + zt1));
+        den = t01 * (t11 * (cs + zt0));
     }
     const double l = dm::log_pos(dm::div(num, den), tab);
     return neg0 ? -l : l;
@@ -181,15 +181,15 @@ __device__ __forceinline__ double gu_sum_paired(double x0, double x1, double y0,
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
         const double c = i ? x1 : x0, cs = i ? xs1 : xs0;
-        const double l = log_quad(y0, y1, r[i][0][0], r[i][0][1], r[i][1][0], r[i][1][1],
-                                  cs + zt0, cs + zt1, tab);
+        const double l = log_quad(y0, y1, r[i][0][0], r[i][0][1], r[i][1][0], r[i][1][1], cs,
+                                  zt0, zt1, tab);
         sum = fma(i ? -c : c, l, sum);
     }
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
         const double c = j ? y1 : y0, cs = j ? ys1 : ys0;
-        const double l = log_quad(x0, x1, r[0][j][0], r[0][j][1], r[1][j][0], r[1][j][1],
-                                  cs + zt0, cs + zt1, tab);
+        const double l = log_quad(x0, x1, r[0][j][0], r[0][j][1], r[1][j][0], r[1][j][1], cs,
+                                  zt0, zt1, tab);
         sum = fma(j ? -c : c, l, sum);
     }
     const bool inside = x0 > 0.0 && x1 < 0.0 && y0 > 0.0 && y1 < 0.0;
