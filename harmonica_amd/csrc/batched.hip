@@ -30,6 +30,8 @@ struct alignas(16) BPrism {
     double w, e, s, n, b, t, grho, pad;
 };
 
+static_assert(kTile <= 128, "tile_pairs marks at most 128 prisms");
+
 __global__ __launch_bounds__(kObsPerBlock) void batched_gz_kernel(
     const double *__restrict__ oe, const double *__restrict__ on, const double *__restrict__ ou,
     const int64_t *__restrict__ obs_off, const double *__restrict__ prisms,
@@ -64,11 +66,7 @@ __global__ __launch_bounds__(kObsPerBlock) void batched_gz_kernel(
             }
             for (int q = threadIdx.x; q < cnt; q += blockDim.x) tile[q].grho = dev::kG * density[t0 + q];
             __syncthreads();
-            for (int q = 0; q < cnt; ++q) {
-                const BPrism sp = tile[q];
-                dev::prism_pair<0x8u>(e, n, u, sp.w, sp.e, sp.s, sp.n, sp.b, sp.t, sp.grho, acc,
-                                      sflags, tables);
-            }
+            dev::tile_pairs<0x8u, false>(tile, cnt, e, n, u, acc, sflags, tables);
         }
         partial[slot * kObsPerBlock + threadIdx.x] = valid ? acc[0] : 0.0;
     }

@@ -86,6 +86,8 @@ struct LayerSource {
     static constexpr bool may_skip = true;
 };
 
+static_assert(kTile <= 128, "tile_pairs marks at most 128 prisms");
+
 template <unsigned M, class Src>
 __global__ __launch_bounds__(kObsPerBlock) void prism_forward_kernel(
     const double *__restrict__ oe, const double *__restrict__ on, const double *__restrict__ ou,
@@ -110,12 +112,7 @@ __global__ __launch_bounds__(kObsPerBlock) void prism_forward_kernel(
         __syncthreads();
         src.stage(tile, t0, cnt);
         __syncthreads();
-        for (int p = 0; p < cnt; ++p) {
-            const StagedPrism sp = tile[p];
-            if (Src::may_skip && sp.skip != 0.0) continue;
-            dev::prism_pair<M>(e, n, u, sp.w, sp.e, sp.s, sp.n, sp.b, sp.t, sp.grho, acc, sflags,
-                               tables);
-        }
+        dev::tile_pairs<M, Src::may_skip>(tile, cnt, e, n, u, acc, sflags, tables);
     }
     if (valid) {
 #pragma unroll

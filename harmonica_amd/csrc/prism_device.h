@@ -70,42 +70,50 @@ struct FieldSet {
 // sum_k (-1)^k ln(P(0,k) / P(1,k)) for four safe_log arguments
 // P(j,k) = a_j + r_jk if a_j >= 0, else w2_k / (r_jk - a_j), where a is the
 // logarithm's own coordinate (a_0 >= a_1) and w2_k = cs + z_k^2 the sum of
-// squares of the other two (passed as cs and zt_k = z_k^2 + 2^-1000).  With t_jk = r_jk + |a_j| (no cancellation), P = t if a >= 0
-// and w2 / t if a < 0, so the four logarithms reduce to one:
+// squares of the other two (passed as cs and zt_k = z_k^2 + 2^-1000).  With
+// t_jk = r_jk + |a_j| (no cancellation), P = t if a >= 0 and w2 / t if
+// a < 0, so the four logarithms reduce to one:
 //   a_1 >= 0:         ln(t00 t11 / (t10 t01))
 //   a_0 < 0:         -ln(t00 t11 / (t10 t01))
-//   a_0 >= 0 > a_1:   ln(t00 t10 w2_1 / (t01 t11 w2_0)).
+//   a_0 >= 0 > a_1:   ln(t00 t10 w2_1 / (t01 t11 w2_0))   (kGeneral only:
+//                     without it the caller guarantees this case is absent).
 // Every t is at least 2^-500 and w2 (with its 2^-1000) at least 2^-1000,
 // so for a prism of nonzero extent the quotient is a positive normal number
 // and the logarithm finite even where the true argument vanishes -- which
 // happens only where the logarithm's coefficient is zero, so that term is
 // exactly 0 without a test.
+template <bool kGeneral>
 __device__ __forceinline__ double log_quad(double a0, double a1, double r00, double r01,
                                            double r10, double r11, double cs, double zt0,
                                            double zt1, const dm::Tables &tab) {
-    const bool neg0 = a0 < 0.0, mixed = !neg0 && a1 < 0.0;
+    const bool neg0 = a0 < 0.0;
     const double t00 = r00 + fabs(a0), t01 = r01 + fabs(a0);
     const double t10 = r10 + fabs(a1), t11 = r11 + fabs(a1);
     double num = t00 * t11, den = t10 * t01;
-    if (__builtin_expect(mixed, 0)) {  // the point lies between the two a-planes
-        num = t00 * (t10 * (cs + zt1));
-        den = t01 * (t11 * (cs + zt0));
+    if constexpr (kGeneral) {
+        const bool mixed = !neg0 && a1 < 0.0;  // the point lies between the two a-planes
+        num = mixed ? t00 * (t10 * (cs + zt1)) : num;
+        den = mixed ? t01 * (t11 * (cs + zt0)) : den;
     }
     const double l = dm::log_pos(dm::div(num, den), tab);
     return neg0 ? -l : l;
 }
 
-// sum_{i,j} (-1)^(i+j) A_z(i, j) on the face at height z != 0 (r_ij the
-// vertex distances): the signed solid angle of the face rectangle.  With
+// sum_{i,j} (-1)^(i+j) A_z(i, j) on the face at height z (r_ij the vertex
+// distances): the signed solid angle of the face rectangle.  With
 // a_i = x_i y / (z r_i) for one y, tan(A_0 - A_1) = (a_0 - a_1) / (1 + a_0 a_1)
 // and 1 + a_0 a_1 has the sign of cos(A_0 - A_1) (cos A_i > 0), so each
 // difference over i is one two-argument angle (N_j, D_j) (both scaled by
 // z^2 r_0 r_1 > 0), and the difference of the two over j is again one angle
 // (N_0 D_1 - N_1 D_0, D_0 D_1 + N_0 N_1), correct modulo 2 pi.  The solid
-// angle lies in (-pi, pi) unless the point projects strictly inside the
-// rectangle, where it has the sign of z and magnitude up to 2 pi; that case
-// takes the branch with the sign of z.  For z = 0 the result is 0 (the
-// angle pair is (0, D >= 0)).
+// angle lies in (-pi, pi) unless the point projects inside the rectangle,
+// where it has the sign of z and magnitude up to 2 pi.  With kWrap, `inside`
+// (the point projects into the rectangle or onto its boundary) selects the
+// branch with the sign of z; a point on the boundary never needs it unless
+// rounding crossed pi.  Without kWrap the caller guarantees the point does
+// not project into the rectangle.  For z = 0 the result is 0 (the angle
+// pair is (0, D >= 0)).
+template <bool kWrap>
 __device__ __forceinline__ double face_angle(double x0, double x1, double y0, double y1, double z,
                                              double r00, double r01, double r10, double r11,
                                              bool inside, const dm::Tables &tab) {
@@ -114,9 +122,12 @@ __device__ __forceinline__ double face_angle(double x0, double x1, double y0, do
     const double n1 = y1 * z * fma(x0, r11, -x1 * r01);
     const double d1 = fma(z * z, r01 * r11, x0 * x1 * (y1 * y1));
     const double a = dm::atan2_full(fma(n0, d1, -n1 * d0), fma(d0, d1, n0 * n1), tab);
-    constexpr double kTwoPi = 6.283185307179586232;
-    const bool wrap = inside && a * z < 0.0;
-    return wrap ? a + copysign(kTwoPi, z) : a;
+    if constexpr (kWrap) {
+        constexpr double kTwoPi = 6.283185307179586232;
+        const bool wrap = inside && a * z < 0.0;
+        return wrap ? a + copysign(kTwoPi, z) : a;
+    }
+    return a;
 }
 
 // ln(P_0 / P_1) for two safe_log arguments that differ only in the
@@ -126,29 +137,49 @@ __device__ __forceinline__ double face_angle(double x0, double x1, double y0, do
 //   a_1 >= 0: ln(t_0 / t_1);  a_0 < 0: -ln(t_0 / t_1);  else ln(t_0 t_1 / w2).
 // On the line of an edge (w2 = 0) safe_log's arguments are 2a (a > 0) or 0
 // (a <= 0, logarithm 0): ln(t_0 / t_1) if a_1 > 0, ln(t_0) if only a_0 > 0,
-// else 0.  The result is finite for every input.
+// else 0.  The result is finite for every input.  Without kGeneral the
+// caller guarantees w2 != 0 and that the point is not between the two
+// a-planes (a_0 >= 0 > a_1).
+template <bool kGeneral>
 __device__ __forceinline__ double log_diff(double a0, double a1, double r0, double r1, double w2,
                                            const dm::Tables &tab) {
     const double t0 = r0 + fabs(a0), t1 = r1 + fabs(a1);
-    const bool line = w2 == 0.0;
-    const bool mixed = a0 >= 0.0 && a1 < 0.0;
-    const double num = line ? (a0 > 0.0 ? t0 : 1.0) : (mixed ? t0 * t1 : t0);
-    const double den = line ? (a1 > 0.0 ? t1 : 1.0) : (mixed ? w2 : t1);
+    const bool neg0 = a0 < 0.0;
+    double num = t0, den = t1;
+    bool flip = neg0;
+    if constexpr (kGeneral) {
+        const bool line = w2 == 0.0, mixed = !neg0 && a1 < 0.0;
+        num = line ? (a0 > 0.0 ? t0 : 1.0) : (mixed ? t0 * t1 : t0);
+        den = line ? (a1 > 0.0 ? t1 : 1.0) : (mixed ? w2 : t1);
+        flip = neg0 && !line;
+    }
     const double l = dm::log_pos(dm::div(num, den), tab);
-    return (a0 < 0.0 && !line) ? -l : l;
+    return flip ? -l : l;
 }
 
 // Signed solid angle sum_{i,j} (-1)^(i+j) atan(a_i b_j / (c r_ij)) of the
 // face at height c, including c = 0 where safe_atan gives +-pi/2 by sign.
+// Without kGeneral the caller guarantees c != 0 and that the point does
+// not project into the face (a_0 >= 0 > a_1 and b_0 >= 0 > b_1 not both).
+template <bool kGeneral>
 __device__ __forceinline__ double face_sum(double a0, double a1, double b0, double b1, double c,
                                            double r00, double r01, double r10, double r11,
                                            const dm::Tables &tab) {
-    const bool inside = a0 > 0.0 && a1 < 0.0 && b0 > 0.0 && b1 < 0.0;
-    const double w = face_angle(a0, a1, b0, b1, c, r00, r01, r10, r11, inside, tab);
-    const double sa = (double)((a0 > 0.0) - (a0 < 0.0) - (a1 > 0.0) + (a1 < 0.0));
-    const double sb = (double)((b0 > 0.0) - (b0 < 0.0) - (b1 > 0.0) + (b1 < 0.0));
-    return c == 0.0 ? 1.5707963267948966 * sa * sb : w;
+    if constexpr (!kGeneral) {
+        return face_angle<false>(a0, a1, b0, b1, c, r00, r01, r10, r11, false, tab);
+    } else {
+        const bool inside = a0 >= 0.0 && a1 < 0.0 && b0 >= 0.0 && b1 < 0.0;
+        const double w = face_angle<true>(a0, a1, b0, b1, c, r00, r01, r10, r11, inside, tab);
+        const double sa = (double)((a0 > 0.0) - (a0 < 0.0) - (a1 > 0.0) + (a1 < 0.0));
+        const double sb = (double)((b0 > 0.0) - (b0 < 0.0) - (b1 > 0.0) + (b1 < 0.0));
+        return c == 0.0 ? 1.5707963267948966 * sa * sb : w;
+    }
 }
+
+// True on every lane of the wave if any lane has `need` (the lean
+// instantiations below are valid only where `need` is false on every lane;
+// choosing per wave keeps the branch uniform, so only one path executes).
+__device__ __forceinline__ bool any_lane(bool need) { return __any(need) != 0; }
 
 // Sum over the 8 vertices of (-1)^(i+j+k) K_u, K_u = -(x L_y + y L_x - z A_z),
 // with the vertices grouped:
@@ -162,10 +193,11 @@ __device__ __forceinline__ double face_sum(double a0, double a1, double b0, doub
 // per-vertex form (their logarithms and angles are finite, see log_quad and
 // face_angle).  With kTop, *ftop receives the top-face sum
 // sum_{i,j} (-1)^(i+j) A_z(i, j, 0) (the derivative with respect to the top).
-template <bool kTop>
-__device__ __forceinline__ double gu_sum_paired(double x0, double x1, double y0, double y1,
-                                                double z0, double z1, const dm::Tables &tab,
-                                                double *ftop) {
+// Without kGeneral the caller guarantees the point lies between neither the
+// two x-planes nor the two y-planes (no mixed quotient, no 2 pi branch).
+template <bool kTop, bool kGeneral>
+__device__ __forceinline__ double gu_sum(double x0, double x1, double y0, double y1, double z0,
+                                         double z1, const dm::Tables &tab, double *ftop) {
     const double xs0 = x0 * x0, xs1 = x1 * x1, ys0 = y0 * y0, ys1 = y1 * y1;
     const double zs0 = z0 * z0, zs1 = z1 * z1;
     const double zt0 = z_tiny(zs0), zt1 = z_tiny(zs1);
@@ -181,23 +213,24 @@ __device__ __forceinline__ double gu_sum_paired(double x0, double x1, double y0,
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
         const double c = i ? x1 : x0, cs = i ? xs1 : xs0;
-        const double l = log_quad(y0, y1, r[i][0][0], r[i][0][1], r[i][1][0], r[i][1][1], cs,
-                                  zt0, zt1, tab);
+        const double l = log_quad<kGeneral>(y0, y1, r[i][0][0], r[i][0][1], r[i][1][0],
+                                            r[i][1][1], cs, zt0, zt1, tab);
         sum = fma(i ? -c : c, l, sum);
     }
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
         const double c = j ? y1 : y0, cs = j ? ys1 : ys0;
-        const double l = log_quad(x0, x1, r[0][j][0], r[0][j][1], r[1][j][0], r[1][j][1], cs,
-                                  zt0, zt1, tab);
+        const double l = log_quad<kGeneral>(x0, x1, r[0][j][0], r[0][j][1], r[1][j][0],
+                                            r[1][j][1], cs, zt0, zt1, tab);
         sum = fma(j ? -c : c, l, sum);
     }
-    const bool inside = x0 > 0.0 && x1 < 0.0 && y0 > 0.0 && y1 < 0.0;
+    // The point projects into the face or onto its boundary (see face_angle).
+    const bool inside = x0 >= 0.0 && x1 < 0.0 && y0 >= 0.0 && y1 < 0.0;
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
         const double z = k ? z1 : z0;
-        const double w = face_angle(x0, x1, y0, y1, z, r[0][0][k], r[0][1][k], r[1][0][k],
-                                    r[1][1][k], inside, tab);
+        const double w = face_angle<kGeneral>(x0, x1, y0, y1, z, r[0][0][k], r[0][1][k],
+                                              r[1][0][k], r[1][1][k], inside, tab);
         sum = fma(k ? z : -z, w, sum);  // w = 0 for z = 0
         if constexpr (kTop) {
             if (k == 0) {
@@ -212,33 +245,27 @@ __device__ __forceinline__ double gu_sum_paired(double x0, double x1, double y0,
     return -sum;
 }
 
-// Add the fields of one prism (bounds p[0..5] = W, E, S, N, B, T; grho =
-// G * density) at (e, n, u) to acc[]; OR singular-point bits into *sflags.
-template <unsigned M>
-__device__ __forceinline__ void prism_pair(double e, double n, double u, double W, double E,
-                                           double S, double N, double B, double T, double grho,
-                                           double (&acc)[FieldSet<M>::count], uint32_t &sflags,
-                                           const dm::Tables &tab) {
+// gu_sum with the lean instantiation unless a lane of the wave lies between
+// the x-planes or the y-planes of the prism.
+template <bool kTop>
+__device__ __forceinline__ double gu_sum_paired(double x0, double x1, double y0, double y1,
+                                                double z0, double z1, const dm::Tables &tab,
+                                                double *ftop) {
+    if (any_lane((x0 >= 0.0 && x1 < 0.0) || (y0 >= 0.0 && y1 < 0.0)))
+        return gu_sum<kTop, true>(x0, x1, y0, y1, z0, z1, tab, ftop);
+    return gu_sum<kTop, false>(x0, x1, y0, y1, z0, z1, tab, ftop);
+}
+
+// The fields of FieldSet<M> of one prism (relative bounds x0 = E - e,
+// x1 = W - e, y0 = N - n, y1 = S - n, z0 = T - u, z1 = B - u) into vs[],
+// singular-point bits into sflags.  Without kGeneral the caller guarantees
+// that the point is on no face plane of the prism and between the two
+// planes of no axis.
+template <unsigned M, bool kGeneral>
+__device__ __forceinline__ void prism_fields(double x0, double x1, double y0, double y1,
+                                             double z0, double z1, double (&vs)[FieldSet<M>::count],
+                                             uint32_t &sflags, const dm::Tables &tab) {
     using F = FieldSet<M>;
-    if constexpr (M == 0x8u) {  // g_u alone
-        acc[0] += grho * gu_sum_paired<false>(E - e, W - e, N - n, S - n, T - u, B - u, tab,
-                                              nullptr);
-        return;
-    }
-    if constexpr (M == 0x2u) {  // g_e alone: K_e is K_u with (x, y, z) -> (y, z, x)
-        acc[0] += grho * gu_sum_paired<false>(N - n, S - n, T - u, B - u, E - e, W - e, tab,
-                                              nullptr);
-        return;
-    }
-    if constexpr (M == 0x4u) {  // g_n alone: K_n is K_u with (x, y, z) -> (z, x, y)
-        acc[0] += grho * gu_sum_paired<false>(T - u, B - u, E - e, W - e, N - n, S - n, tab,
-                                              nullptr);
-        return;
-    }
-    double vs[F::count];
-    const double x0 = E - e, x1 = W - e;
-    const double y0 = N - n, y1 = S - n;
-    const double z0 = T - u, z1 = B - u;
     const double xs[2] = {x0 * x0, x1 * x1}, ys[2] = {y0 * y0, y1 * y1};
     const double zs[2] = {z0 * z0, z1 * z1};
     const double X[2] = {x0, x1}, Y[2] = {y0, y1}, Z[2] = {z0, z1};
@@ -261,20 +288,23 @@ __device__ __forceinline__ void prism_pair(double e, double n, double u, double 
 #pragma unroll
         for (int b = 0; b < 2; ++b) {
             if constexpr (F::Lx)
-                DLx[a][b] = log_diff(x0, x1, r[0][a][b], r[1][a][b], ys[a] + zs[b], tab);
+                DLx[a][b] = log_diff<kGeneral>(x0, x1, r[0][a][b], r[1][a][b], ys[a] + zs[b], tab);
             if constexpr (F::Ly)
-                DLy[a][b] = log_diff(y0, y1, r[a][0][b], r[a][1][b], xs[a] + zs[b], tab);
+                DLy[a][b] = log_diff<kGeneral>(y0, y1, r[a][0][b], r[a][1][b], xs[a] + zs[b], tab);
             if constexpr (F::Lz)
-                DLz[a][b] = log_diff(z0, z1, r[a][b][0], r[a][b][1], xs[a] + ys[b], tab);
+                DLz[a][b] = log_diff<kGeneral>(z0, z1, r[a][b][0], r[a][b][1], xs[a] + ys[b], tab);
         }
 #pragma unroll
     for (int a = 0; a < 2; ++a) {
         if constexpr (F::Ax)
-            Ox[a] = face_sum(y0, y1, z0, z1, X[a], r[a][0][0], r[a][0][1], r[a][1][0], r[a][1][1], tab);
+            Ox[a] = face_sum<kGeneral>(y0, y1, z0, z1, X[a], r[a][0][0], r[a][0][1], r[a][1][0],
+                                       r[a][1][1], tab);
         if constexpr (F::Ay)
-            Oy[a] = face_sum(z0, z1, x0, x1, Y[a], r[0][a][0], r[1][a][0], r[0][a][1], r[1][a][1], tab);
+            Oy[a] = face_sum<kGeneral>(z0, z1, x0, x1, Y[a], r[0][a][0], r[1][a][0], r[0][a][1],
+                                       r[1][a][1], tab);
         if constexpr (F::Az)
-            Oz[a] = face_sum(x0, x1, y0, y1, Z[a], r[0][0][a], r[0][1][a], r[1][0][a], r[1][1][a], tab);
+            Oz[a] = face_sum<kGeneral>(x0, x1, y0, y1, Z[a], r[0][0][a], r[0][1][a], r[1][0][a],
+                                       r[1][1][a], tab);
     }
     // Signed sums with one coefficient (or none) per grouped quantity.
     auto s2 = [](const double (&d)[2][2], double c0, double c1, bool by_first) {
@@ -302,9 +332,13 @@ __device__ __forceinline__ void prism_pair(double e, double n, double u, double 
     if constexpr (F::en) vs[F::slot(7)] = s2(DLz, 1.0, 1.0, true);
     if constexpr (F::eu) vs[F::slot(8)] = s2(DLy, 1.0, 1.0, true);
     if constexpr (F::nu) vs[F::slot(9)] = s2(DLx, 1.0, 1.0, true);
-    if constexpr (F::tensor) {
-        const bool in_e = W <= e && e <= E, in_n = S <= n && n <= N, in_u = B <= u && u <= T;
-        const bool at_e = e == W || e == E, at_n = n == S || n == N, at_u = u == B || u == T;
+    if constexpr (F::tensor && kGeneral) {
+        // Singular points (gravity.py:141-158): on an edge of the prism,
+        // in x = E - e etc. (x0 = 0 is e == E, x1 <= 0 <= x0 is W <= e <= E).
+        const bool in_e = x1 <= 0.0 && 0.0 <= x0, in_n = y1 <= 0.0 && 0.0 <= y0;
+        const bool in_u = z1 <= 0.0 && 0.0 <= z0;
+        const bool at_e = x0 == 0.0 || x1 == 0.0, at_n = y0 == 0.0 || y1 == 0.0;
+        const bool at_u = z0 == 0.0 || z1 == 0.0;
         const bool edge_e = in_e && at_n && at_u;  // on an easting-parallel edge
         const bool edge_n = in_n && at_e && at_u;  // on a northing-parallel edge
         const bool edge_u = in_u && at_e && at_n;  // on an upward edge
@@ -322,8 +356,102 @@ __device__ __forceinline__ void prism_pair(double e, double n, double u, double 
         if constexpr (F::eu) sing(8, edge_n);
         if constexpr (F::nu) sing(9, edge_e);
     }
+}
+
+// Whether the point (e, n, u) needs the general instantiation for this
+// prism: for g_u, g_e, g_n alone when it lies between the two planes of one
+// of the two axes the logarithms run along; for other field sets when it
+// lies between the two planes of any axis or on a face plane (a product of
+// the two differences that is 0, which includes underflow: the general
+// path is exact everywhere).
+template <unsigned M>
+__device__ __forceinline__ bool needs_general(double e, double n, double u, double W, double E,
+                                              double S, double N, double B, double T) {
+    const double x0 = E - e, x1 = W - e, y0 = N - n, y1 = S - n, z0 = T - u, z1 = B - u;
+    const bool bx = x0 >= 0.0 && x1 < 0.0, by = y0 >= 0.0 && y1 < 0.0;
+    const bool bz = z0 >= 0.0 && z1 < 0.0;
+    if constexpr (M == 0x8u) return bx || by;
+    if constexpr (M == 0x2u) return by || bz;
+    if constexpr (M == 0x4u) return bz || bx;
+    return bx || by || bz || x0 * x1 == 0.0 || y0 * y1 == 0.0 || z0 * z1 == 0.0;
+}
+
+// Add the fields of one prism (bounds W, E, S, N, B, T; grho = G * density)
+// at (e, n, u) to acc[]; OR singular-point bits into sflags.  Without
+// kGeneral, needs_general<M> must be false for the point.
+template <unsigned M, bool kGeneral>
+__device__ __forceinline__ void prism_pair_path(double e, double n, double u, double W, double E,
+                                                double S, double N, double B, double T,
+                                                double grho, double (&acc)[FieldSet<M>::count],
+                                                uint32_t &sflags, const dm::Tables &tab) {
+    using F = FieldSet<M>;
+    if constexpr (M == 0x8u) {  // g_u alone
+        acc[0] += grho * gu_sum<false, kGeneral>(E - e, W - e, N - n, S - n, T - u, B - u, tab,
+                                                 nullptr);
+    } else if constexpr (M == 0x2u) {  // g_e alone: K_e is K_u with (x, y, z) -> (y, z, x)
+        acc[0] += grho * gu_sum<false, kGeneral>(N - n, S - n, T - u, B - u, E - e, W - e, tab,
+                                                 nullptr);
+    } else if constexpr (M == 0x4u) {  // g_n alone: K_n is K_u with (x, y, z) -> (z, x, y)
+        acc[0] += grho * gu_sum<false, kGeneral>(T - u, B - u, E - e, W - e, N - n, S - n, tab,
+                                                 nullptr);
+    } else {
+        double vs[F::count];
+        prism_fields<M, kGeneral>(E - e, W - e, N - n, S - n, T - u, B - u, vs, sflags, tab);
 #pragma unroll
-    for (int s = 0; s < F::count; ++s) acc[s] += grho * vs[s];
+        for (int s = 0; s < F::count; ++s) acc[s] += grho * vs[s];
+    }
+}
+
+// prism_pair_path with the instantiation chosen per prism and wave.
+template <unsigned M>
+__device__ __forceinline__ void prism_pair(double e, double n, double u, double W, double E,
+                                           double S, double N, double B, double T, double grho,
+                                           double (&acc)[FieldSet<M>::count], uint32_t &sflags,
+                                           const dm::Tables &tab) {
+    if (any_lane(needs_general<M>(e, n, u, W, E, S, N, B, T)))
+        prism_pair_path<M, true>(e, n, u, W, E, S, N, B, T, grho, acc, sflags, tab);
+    else
+        prism_pair_path<M, false>(e, n, u, W, E, S, N, B, T, grho, acc, sflags, tab);
+}
+
+// Accumulate the prisms tile[0, cnt) (LDS, read as wave-uniform broadcasts;
+// P has w, e, s, n, b, t, grho and, with kMaySkip, skip) at (e, n, u).
+// Pass 1 evaluates the lean form for every prism that no lane of the wave
+// needs the general form for and marks the others in a wave-uniform
+// 128-bit mask; pass 2 evaluates the general form for the marked prisms.
+// Two loops keep the two forms' register allocations apart.  The order of
+// the additions is fixed by the prisms and the points, so results are
+// deterministic.
+template <unsigned M, bool kMaySkip, class P>
+__device__ __forceinline__ void tile_pairs(const P *tile, int cnt, double e, double n, double u,
+                                           double (&acc)[FieldSet<M>::count], uint32_t &sflags,
+                                           const dm::Tables &tab) {
+    uint64_t general_lo = 0, general_hi = 0;
+    for (int p = 0; p < cnt; ++p) {
+        const P sp = tile[p];
+        if constexpr (kMaySkip) {
+            if (sp.skip != 0.0) continue;
+        }
+        if (any_lane(needs_general<M>(e, n, u, sp.w, sp.e, sp.s, sp.n, sp.b, sp.t))) {
+            if (p < 64)
+                general_lo |= 1ull << p;
+            else
+                general_hi |= 1ull << (p - 64);
+            continue;
+        }
+        prism_pair_path<M, false>(e, n, u, sp.w, sp.e, sp.s, sp.n, sp.b, sp.t, sp.grho, acc,
+                                  sflags, tab);
+    }
+    for (int half = 0; half < 2; ++half) {
+        uint64_t m = half ? general_hi : general_lo;
+        while (m) {
+            const int p = 64 * half + __builtin_ctzll(m);
+            m &= m - 1;
+            const P sp = tile[p];
+            prism_pair_path<M, true>(e, n, u, sp.w, sp.e, sp.s, sp.n, sp.b, sp.t, sp.grho, acc,
+                                     sflags, tab);
+        }
+    }
 }
 
 // Harmonica's sign flip and unit conversion of field f (gravity.py:228-235).
