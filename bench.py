@@ -35,22 +35,23 @@ FP64_PEAK = 78.6e12
 # FP64 operations per prism-observation pair executed by the g_z kernel
 # (prism_forward_kernel<g_z>, grouped closed form): rocprofv3 counters
 # SQ_INSTS_VALU_{ADD,MUL,TRANS}_F64 + 2 x SQ_INSTS_VALU_FMA_F64, x 64 lanes,
-# divided by the pairs of the profiled launch
-# (profiles/prism_gz_terrain1M_counters.md).  The same profile gives 347.8
-# VALU instructions per pair, 16 of them FP64 transcendentals (v_rsq_f64,
-# v_rcp_f64).  A wave64 VALU instruction occupies a SIMD for 4 cycles and
+# divided by the pairs of the launch, on this workload
+# (profiles/prism_gz_terrain1M_counters.md).  The same profile gives 322.1
+# VALU instructions per pair, 14.27 of them FP64 transcendentals (v_rsq_f64,
+# v_rcp_f64; 16 per pair less the square roots of bottom vertices shared
+# with the previous prism).  A wave64 VALU instruction occupies a SIMD for 4 cycles and
 # an FP64 transcendental for 16 (measured on the MI355X), which bounds the
 # pair rate at 1024 SIMDs x 2.4 GHz x 64 lanes / (4 (VALU - TRANS) + 16
 # TRANS) cycles.  The per-vertex form of the same closed form (8 square
 # roots, 16 logarithms, 8 arctangents per pair) measured 1083.7 FP64
 # operations per pair; it is reported beside the executed count.
-FLOPS_PER_PAIR_GZ = 394.6
-VALU_INSTR_PER_PAIR_GZ = 347.8
-TRANS_PER_PAIR_GZ = 16.0
+FLOPS_PER_PAIR_GZ = 368.2
+VALU_INSTR_PER_PAIR_GZ = 322.1
+TRANS_PER_PAIR_GZ = 14.27
 PER_VERTEX_FLOPS_PER_PAIR_GZ = 1083.7
 # Same counters for the ten fields fused (prism_forward_kernel<all>).
-FLOPS_PER_PAIR_ALL = 996.0
-VALU_INSTR_PER_PAIR_ALL = 1124.8
+FLOPS_PER_PAIR_ALL = 953.6
+VALU_INSTR_PER_PAIR_ALL = 825.1
 TRANS_PER_PAIR_ALL = 32.0
 SIMD_CYCLES_PER_S = 1024 * 2.4e9
 
@@ -62,9 +63,9 @@ def valu_issue_frac(pair_rate, valu, trans):
 
 
 # HBM bytes per launch of the g_z kernel on this workload: FETCH_SIZE
-# 219594 KiB x 2 (a 1 GiB copy reads as 524308 KiB) + WRITE_SIZE 20480 KiB
+# 219639 KiB x 2 (a 1 GiB copy reads as 524308 KiB) + WRITE_SIZE 20480 KiB
 # x 1 (profiles/prism_gz_terrain1M_counters.md).
-TRAFFIC_BYTES_GZ = (219594.0 * 2 + 20480.0) * 1024
+TRAFFIC_BYTES_GZ = (219639.0 * 2 + 20480.0) * 1024
 
 
 def log(msg):

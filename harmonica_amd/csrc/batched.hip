@@ -27,7 +27,8 @@ namespace {
 constexpr int kItemWords = 5;  // model, tile, p0, p1, slot
 
 struct alignas(16) BPrism {
-    double w, e, s, n, b, t, grho, pad;
+    double w, e, s, n, b, t, grho;
+    int skip, share;  // never set: batched models are not marked for edge sharing
 };
 
 static_assert(kTile <= 128, "tile_pairs marks at most 128 prisms");
@@ -64,7 +65,11 @@ __global__ __launch_bounds__(kObsPerBlock) void batched_gz_kernel(
                 const int q = idx / 6, c = idx - 6 * q;
                 flat[8 * q + c] = prisms[6 * t0 + idx];
             }
-            for (int q = threadIdx.x; q < cnt; q += blockDim.x) tile[q].grho = dev::kG * density[t0 + q];
+            for (int q = threadIdx.x; q < cnt; q += blockDim.x) {
+                tile[q].grho = dev::kG * density[t0 + q];
+                tile[q].skip = 0;
+                tile[q].share = 0;
+            }
             __syncthreads();
             dev::tile_pairs<0x8u, false>(tile, cnt, e, n, u, acc, sflags, tables);
         }

@@ -15,9 +15,11 @@ namespace {
 
 using dev::FieldSet;
 
-// One staged prism: bounds, G * density, skip flag.
+// One staged prism (64 bytes): bounds, G * density, skip flag and the edge
+// it shares with the previous prism of the tile (dev::shared_edge).
 struct alignas(16) StagedPrism {
-    double w, e, s, n, b, t, grho, skip;
+    double w, e, s, n, b, t, grho;
+    int skip, share;
 };
 
 // Stage prisms [t0, t0 + cnt) of an (m, 6) row-major array.
@@ -32,7 +34,8 @@ __device__ __forceinline__ void stage_prisms(StagedPrism *tile, const double *__
     }
     for (int p = threadIdx.x; p < cnt; p += blockDim.x) {
         tile[p].grho = dev::kG * density[t0 + p];
-        tile[p].skip = 0.0;
+        tile[p].skip = 0;
+        tile[p].share = 0;
     }
 }
 
@@ -63,7 +66,8 @@ __device__ __forceinline__ void stage_layer(StagedPrism *tile, const double *__r
         sp.b = b;
         sp.t = t;
         sp.grho = dev::kG * d;
-        sp.skip = skip ? 1.0 : 0.0;
+        sp.skip = skip ? 1 : 0;
+        sp.share = 0;
         tile[q] = sp;
     }
 }
@@ -112,6 +116,21 @@ __global__ __launch_bounds__(kObsPerBlock) void prism_forward_kernel(
         __syncthreads();
         src.stage(tile, t0, cnt);
         __syncthreads();
+        if constexpr (M == 0x8u) {  // edges shared with the previous prism (g_u reuse)
+            int sh[(kTile + kObsPerBlock - 1) / kObsPerBlock];
+#pragma unroll
+            for (int k = 0; k * kObsPerBlock < kTile; ++k) {
+                const int q = threadIdx.x + k * kObsPerBlock;
+                sh[k] = (q > 0 && q < cnt) ? dev::shared_edge(tile[q - 1], tile[q]) : 0;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k * kObsPerBlock < kTile; ++k) {
+                const int q = threadIdx.x + k * kObsPerBlock;
+                if (q < cnt) tile[q].share = sh[k];
+            }
+            __syncthreads();
+        }
         dev::tile_pairs<M, Src::may_skip>(tile, cnt, e, n, u, acc, sflags, tables);
     }
     if (valid) {

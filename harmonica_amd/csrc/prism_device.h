@@ -82,10 +82,11 @@ struct FieldSet {
 // and the logarithm finite even where the true argument vanishes -- which
 // happens only where the logarithm's coefficient is zero, so that term is
 // exactly 0 without a test.
+// `flip` must be dm::sign_mask(a0 < 0) (shared by the quotients of one axis).
 template <bool kGeneral>
 __device__ __forceinline__ double log_quad(double a0, double a1, double r00, double r01,
                                            double r10, double r11, double cs, double zt0,
-                                           double zt1, const dm::Tables &tab) {
+                                           double zt1, unsigned flip, const dm::Tables &tab) {
     const bool neg0 = a0 < 0.0;
     const double t00 = r00 + fabs(a0), t01 = r01 + fabs(a0);
     const double t10 = r10 + fabs(a1), t11 = r11 + fabs(a1);
@@ -95,8 +96,7 @@ __device__ __forceinline__ double log_quad(double a0, double a1, double r00, dou
         num = mixed ? t00 * (t10 * (cs + zt1)) : num;
         den = mixed ? t01 * (t11 * (cs + zt0)) : den;
     }
-    const double l = dm::log_pos(dm::div(num, den), tab);
-    return neg0 ? -l : l;
+    return dm::flip_sign(dm::log_pos(dm::div(num, den), tab), flip);
 }
 
 // sum_{i,j} (-1)^(i+j) A_z(i, j) on the face at height z (r_ij the vertex
@@ -113,7 +113,9 @@ __device__ __forceinline__ double log_quad(double a0, double a1, double r00, dou
 // rounding crossed pi.  Without kWrap the caller guarantees the point does
 // not project into the rectangle.  For z = 0 the result is 0 (the angle
 // pair is (0, D >= 0)).
-template <bool kWrap>
+// kZeroSafe: z, x_i or y_j may be 0 (then the angle pair may be (0, 0) and
+// the result is 0); without it none of them is.
+template <bool kWrap, bool kZeroSafe = true>
 __device__ __forceinline__ double face_angle(double x0, double x1, double y0, double y1, double z,
                                              double r00, double r01, double r10, double r11,
                                              bool inside, const dm::Tables &tab) {
@@ -121,7 +123,8 @@ __device__ __forceinline__ double face_angle(double x0, double x1, double y0, do
     const double d0 = fma(z * z, r00 * r10, x0 * x1 * (y0 * y0));
     const double n1 = y1 * z * fma(x0, r11, -x1 * r01);
     const double d1 = fma(z * z, r01 * r11, x0 * x1 * (y1 * y1));
-    const double a = dm::atan2_full(fma(n0, d1, -n1 * d0), fma(d0, d1, n0 * n1), tab);
+    const double a =
+        dm::atan2_full<kZeroSafe>(fma(n0, d1, -n1 * d0), fma(d0, d1, n0 * n1), tab);
     if constexpr (kWrap) {
         constexpr double kTwoPi = 6.283185307179586232;
         const bool wrap = inside && a * z < 0.0;
@@ -140,21 +143,19 @@ __device__ __forceinline__ double face_angle(double x0, double x1, double y0, do
 // else 0.  The result is finite for every input.  Without kGeneral the
 // caller guarantees w2 != 0 and that the point is not between the two
 // a-planes (a_0 >= 0 > a_1).
+// `flip` must be dm::sign_mask(a0 < 0) (shared by the quotients of one axis).
 template <bool kGeneral>
 __device__ __forceinline__ double log_diff(double a0, double a1, double r0, double r1, double w2,
-                                           const dm::Tables &tab) {
+                                           unsigned flip, const dm::Tables &tab) {
     const double t0 = r0 + fabs(a0), t1 = r1 + fabs(a1);
-    const bool neg0 = a0 < 0.0;
     double num = t0, den = t1;
-    bool flip = neg0;
     if constexpr (kGeneral) {
-        const bool line = w2 == 0.0, mixed = !neg0 && a1 < 0.0;
+        const bool line = w2 == 0.0, mixed = !(a0 < 0.0) && a1 < 0.0;
         num = line ? (a0 > 0.0 ? t0 : 1.0) : (mixed ? t0 * t1 : t0);
         den = line ? (a1 > 0.0 ? t1 : 1.0) : (mixed ? w2 : t1);
-        flip = neg0 && !line;
+        flip = line ? 0u : flip;
     }
-    const double l = dm::log_pos(dm::div(num, den), tab);
-    return flip ? -l : l;
+    return dm::flip_sign(dm::log_pos(dm::div(num, den), tab), flip);
 }
 
 // Signed solid angle sum_{i,j} (-1)^(i+j) atan(a_i b_j / (c r_ij)) of the
@@ -166,7 +167,7 @@ __device__ __forceinline__ double face_sum(double a0, double a1, double b0, doub
                                            double r00, double r01, double r10, double r11,
                                            const dm::Tables &tab) {
     if constexpr (!kGeneral) {
-        return face_angle<false>(a0, a1, b0, b1, c, r00, r01, r10, r11, false, tab);
+        return face_angle<false, false>(a0, a1, b0, b1, c, r00, r01, r10, r11, false, tab);
     } else {
         const bool inside = a0 >= 0.0 && a1 < 0.0 && b0 >= 0.0 && b1 < 0.0;
         const double w = face_angle<true>(a0, a1, b0, b1, c, r00, r01, r10, r11, inside, tab);
@@ -195,33 +196,63 @@ __device__ __forceinline__ bool any_lane(bool need) { return __any(need) != 0; }
 // sum_{i,j} (-1)^(i+j) A_z(i, j, 0) (the derivative with respect to the top).
 // Without kGeneral the caller guarantees the point lies between neither the
 // two x-planes nor the two y-planes (no mixed quotient, no 2 pi branch).
-template <bool kTop, bool kGeneral>
+template <bool kTop, bool kGeneral, bool kShare = false>
 __device__ __forceinline__ double gu_sum(double x0, double x1, double y0, double y1, double z0,
-                                         double z1, const dm::Tables &tab, double *ftop) {
-    const double xs0 = x0 * x0, xs1 = x1 * x1, ys0 = y0 * y0, ys1 = y1 * y1;
+                                         double z1, const dm::Tables &tab, double *ftop,
+                                         int share = 0, double (*rb)[2] = nullptr) {
+    const double ys0 = y0 * y0, ys1 = y1 * y1;
     const double zs0 = z0 * z0, zs1 = z1 * z1;
     const double zt0 = z_tiny(zs0), zt1 = z_tiny(zs1);
+    // x^2 + y^2 as one fma (x^2 itself is needed only by the general form).
+    const double X[2] = {x0, x1}, YS[2] = {ys0, ys1};
     double r[2][2][2];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
+        for (int j = 0; j < 2; ++j) {
+            const double xy = fma(X[i], X[i], YS[j]);
+            r[i][j][0] = vertex_r(xy, zt0);
+            if constexpr (!kShare) r[i][j][1] = vertex_r(xy, zt1);
+        }
+    if constexpr (kShare) {
+        // Bottom vertices (k = 1).  share 1: this prism's west edge is the
+        // previous prism's east edge (same S, N, B), so r[1][j][1] equals the
+        // previous r[0][j][1]; share 2: its south edge is the previous north
+        // edge (same W, E, B), so r[i][1][1] equals the previous r[i][0][1].
+        // rb holds the previous prism's bottom distances and receives these.
+        r[0][0][1] = vertex_r(fma(x0, x0, ys0), zt1);
+        if (share == 1) {
+            r[1][0][1] = rb[0][0];
+            r[1][1][1] = rb[0][1];
+            r[0][1][1] = vertex_r(fma(x0, x0, ys1), zt1);
+        } else if (share == 2) {
+            r[0][1][1] = rb[0][0];
+            r[1][1][1] = rb[1][0];
+            r[1][0][1] = vertex_r(fma(x1, x1, ys0), zt1);
+        } else {
+            r[0][1][1] = vertex_r(fma(x0, x0, ys1), zt1);
+            r[1][0][1] = vertex_r(fma(x1, x1, ys0), zt1);
+            r[1][1][1] = vertex_r(fma(x1, x1, ys1), zt1);
+        }
 #pragma unroll
-            for (int k = 0; k < 2; ++k)
-                r[i][j][k] = vertex_r((i ? xs1 : xs0) + (j ? ys1 : ys0), k ? zt1 : zt0);
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) rb[i][j] = r[i][j][1];
+    }
     double sum = 0.0, top = 0.0;
+    const unsigned flip_y = dm::sign_mask(y0 < 0.0), flip_x = dm::sign_mask(x0 < 0.0);
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
-        const double c = i ? x1 : x0, cs = i ? xs1 : xs0;
+        const double c = i ? x1 : x0, cs = c * c;  // dead unless kGeneral
         const double l = log_quad<kGeneral>(y0, y1, r[i][0][0], r[i][0][1], r[i][1][0],
-                                            r[i][1][1], cs, zt0, zt1, tab);
+                                            r[i][1][1], cs, zt0, zt1, flip_y, tab);
         sum = fma(i ? -c : c, l, sum);
     }
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
         const double c = j ? y1 : y0, cs = j ? ys1 : ys0;
         const double l = log_quad<kGeneral>(x0, x1, r[0][j][0], r[0][j][1], r[1][j][0],
-                                            r[1][j][1], cs, zt0, zt1, tab);
+                                            r[1][j][1], cs, zt0, zt1, flip_x, tab);
         sum = fma(j ? -c : c, l, sum);
     }
     // The point projects into the face or onto its boundary (see face_angle).
@@ -283,16 +314,21 @@ __device__ __forceinline__ void prism_fields(double x0, double x1, double y0, do
     //   sum (-1)^(i+j+k) f(x_i) A_x = sum_i (-1)^i f(x_i) Ox[i], Ox[i] the
     //   face solid angle (one arctangent), and likewise Oy[j], Oz[k].
     double DLx[2][2] = {}, DLy[2][2] = {}, DLz[2][2] = {}, Ox[2] = {}, Oy[2] = {}, Oz[2] = {};
+    const unsigned fx = dm::sign_mask(x0 < 0.0), fy = dm::sign_mask(y0 < 0.0);
+    const unsigned fz = dm::sign_mask(z0 < 0.0);
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < 2; ++b) {
             if constexpr (F::Lx)
-                DLx[a][b] = log_diff<kGeneral>(x0, x1, r[0][a][b], r[1][a][b], ys[a] + zs[b], tab);
+                DLx[a][b] = log_diff<kGeneral>(x0, x1, r[0][a][b], r[1][a][b], ys[a] + zs[b], fx,
+                                               tab);
             if constexpr (F::Ly)
-                DLy[a][b] = log_diff<kGeneral>(y0, y1, r[a][0][b], r[a][1][b], xs[a] + zs[b], tab);
+                DLy[a][b] = log_diff<kGeneral>(y0, y1, r[a][0][b], r[a][1][b], xs[a] + zs[b], fy,
+                                               tab);
             if constexpr (F::Lz)
-                DLz[a][b] = log_diff<kGeneral>(z0, z1, r[a][b][0], r[a][b][1], xs[a] + ys[b], tab);
+                DLz[a][b] = log_diff<kGeneral>(z0, z1, r[a][b][0], r[a][b][1], xs[a] + ys[b], fz,
+                                               tab);
         }
 #pragma unroll
     for (int a = 0; a < 2; ++a) {
@@ -415,7 +451,8 @@ __device__ __forceinline__ void prism_pair(double e, double n, double u, double 
 }
 
 // Accumulate the prisms tile[0, cnt) (LDS, read as wave-uniform broadcasts;
-// P has w, e, s, n, b, t, grho and, with kMaySkip, skip) at (e, n, u).
+// P has w, e, s, n, b, t, grho, int share and, with kMaySkip, int skip) at
+// (e, n, u).
 // Pass 1 evaluates the lean form for every prism that no lane of the wave
 // needs the general form for and marks the others in a wave-uniform
 // 128-bit mask; pass 2 evaluates the general form for the marked prisms.
@@ -427,20 +464,36 @@ __device__ __forceinline__ void tile_pairs(const P *tile, int cnt, double e, dou
                                            double (&acc)[FieldSet<M>::count], uint32_t &sflags,
                                            const dm::Tables &tab) {
     uint64_t general_lo = 0, general_hi = 0;
+    // g_u alone: bottom vertex distances of the previous prism evaluated in
+    // this pass, reused where P::share says an edge is shared.
+    double rb[2][2];
+    bool prev_lean = false;
     for (int p = 0; p < cnt; ++p) {
         const P sp = tile[p];
         if constexpr (kMaySkip) {
-            if (sp.skip != 0.0) continue;
+            if (sp.skip) {
+                prev_lean = false;
+                continue;
+            }
         }
         if (any_lane(needs_general<M>(e, n, u, sp.w, sp.e, sp.s, sp.n, sp.b, sp.t))) {
             if (p < 64)
                 general_lo |= 1ull << p;
             else
                 general_hi |= 1ull << (p - 64);
+            prev_lean = false;
             continue;
         }
-        prism_pair_path<M, false>(e, n, u, sp.w, sp.e, sp.s, sp.n, sp.b, sp.t, sp.grho, acc,
-                                  sflags, tab);
+        if constexpr (M == 0x8u) {
+            const int share = prev_lean ? __builtin_amdgcn_readfirstlane(sp.share) : 0;
+            acc[0] += sp.grho * gu_sum<false, false, true>(sp.e - e, sp.w - e, sp.n - n, sp.s - n,
+                                                          sp.t - u, sp.b - u, tab, nullptr,
+                                                          share, rb);
+            prev_lean = true;
+        } else {
+            prism_pair_path<M, false>(e, n, u, sp.w, sp.e, sp.s, sp.n, sp.b, sp.t, sp.grho, acc,
+                                      sflags, tab);
+        }
     }
     for (int half = 0; half < 2; ++half) {
         uint64_t m = half ? general_hi : general_lo;
@@ -452,6 +505,16 @@ __device__ __forceinline__ void tile_pairs(const P *tile, int cnt, double e, dou
                                      sflags, tab);
         }
     }
+}
+
+// Edge sharing between tile[q - 1] and tile[q] for the g_u bottom-vertex
+// reuse: 1 if q's west edge is q - 1's east edge (same S, N, B), 2 if q's
+// south edge is q - 1's north edge (same W, E, B), else 0.
+template <class P>
+__device__ __forceinline__ int shared_edge(const P &prev, const P &cur) {
+    if (cur.w == prev.e && cur.s == prev.s && cur.n == prev.n && cur.b == prev.b) return 1;
+    if (cur.s == prev.n && cur.w == prev.w && cur.e == prev.e && cur.b == prev.b) return 2;
+    return 0;
 }
 
 // Harmonica's sign flip and unit conversion of field f (gravity.py:228-235).

@@ -258,6 +258,31 @@ def test_terrain_1m_all_fields_parity(hm):
             assert_parity(got, expected, err_msg=f"prism_gravity {field}")
 
 
+@pytest.mark.parametrize("order", ["easting", "northing"])
+def test_shared_edges_match_unshared(hm, order):
+    """g_z reuses the bottom vertex distances of the previous prism when the
+    two share an edge and a bottom (rows along easting: west edge = previous
+    east edge; along northing: south = previous north).  Same prisms in an
+    order that shares nothing (each pair of neighbours swapped) must agree
+    to summation-order rounding, and both with the oracle."""
+    rng = np.random.default_rng(11)
+    ce = np.arange(24) * 75.0 - 900.0
+    cn = np.arange(20) * 75.0 - 700.0
+    E, N = np.meshgrid(ce, cn, indexing="xy" if order == "easting" else "ij")
+    top = rng.uniform(50.0, 900.0, E.shape)
+    bottom = np.full(E.shape, -1500.0)
+    bottom.flat[rng.choice(bottom.size, 25, replace=False)] = -1499.0  # breaks some sharing
+    prisms = np.column_stack([(E - 37.5).ravel(), (E + 37.5).ravel(), (N - 37.5).ravel(),
+                              (N + 37.5).ravel(), bottom.ravel(), top.ravel()])
+    dens = rng.uniform(2000.0, 3000.0, len(prisms))
+    coords = grid_coordinates((-1500, 1500, -1200, 1300), shape=(17, 23), extra=1200.0)
+    shared = hm.prism_gravity(coords, prisms, dens, field="g_z")
+    swap = np.arange(len(prisms)).reshape(-1, 2)[:, ::-1].ravel()
+    unshared = hm.prism_gravity(coords, prisms[swap], dens[swap], field="g_z")
+    npt.assert_allclose(shared, unshared, rtol=0, atol=1e-13 * np.abs(unshared).max())
+    assert_parity(shared, oracle.prism_gravity(coords, prisms, dens, "g_z"))
+
+
 # ----------------------------------------------------------- point masses
 @pytest.mark.parametrize("field", FIELDS + ("g_ne", "g_ze", "g_zn"))
 def test_point_cartesian_vs_oracle(hm, field):
