@@ -187,6 +187,24 @@ def predict_device(coords, points, coefs, dtype=torch.float64):
     return out
 
 
+def spatial_order(easting, northing, bits=16):
+    """Permutation that sorts points along a Morton (Z-order) curve of their
+    horizontal coordinates, so that runs of consecutive points are compact
+    regions."""
+    def cells(v):
+        v = np.asarray(v, dtype=np.float64)
+        span = v.max() - v.min()
+        q = (v - v.min()) / (span if span > 0 else 1.0) * (2**bits - 1)
+        return q.astype(np.uint64)
+
+    x, y = cells(easting), cells(northing)
+    code = np.zeros_like(x)
+    for b in range(bits):
+        code |= ((x >> np.uint64(b)) & np.uint64(1)) << np.uint64(2 * b)
+        code |= ((y >> np.uint64(b)) & np.uint64(1)) << np.uint64(2 * b + 1)
+    return np.argsort(code, kind="stable")
+
+
 def fit_device(coords, data, weights, points, damping, precision="float64", max_iter=None,
                rtol=None):
     """Coefficients of the scaled, damped, weighted least-squares fit on the
@@ -206,6 +224,16 @@ def fit_device(coords, data, weights, points, damping, precision="float64", max_
         max_iter = 10 if mode == _lib.HMX_FIT_FP64 else 300
     if rtol is None:
         rtol = 1e-12 if mode == _lib.HMX_FIT_FP64 else 1e-9
+    if mode == _lib.HMX_FIT_BF16_SPLIT:
+        # The normal equations sum over the data, so their order is free.  In
+        # spatial order each group of 1024 data the split-bf16 normal matrix
+        # scales separately is a compact region, where the columns of far
+        # sources are uniformly small: they keep more significant bits, and
+        # the matrix preconditions better.
+        order = spatial_order(coords[0], coords[1])
+        coords = tuple(np.asarray(x)[order] for x in coords)
+        data = np.asarray(data)[order]
+        weights = None if weights is None else np.asarray(weights)[order]
     c = [_dev(x, torch.float64) for x in coords]
     p = [_dev(x, torch.float64) for x in points]
     d = _dev(data, torch.float64)

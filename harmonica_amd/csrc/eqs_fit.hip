@@ -715,33 +715,49 @@ __global__ void zero_int_kernel(int *p) {
 }
 
 // ------------------------------------------ split-precision normal matrix
-// Exact-product scheme for X^T X on bf16 MFMA.  Column j of
-// X = W^1/2 J S is scaled by f_j = 2^(E_j + 1), where max_k |X_kj| < 2^E_j,
-// so x = X_kj / f_j lies in (-1/2, 1/2), and cut into kSlices integer
-// slices: sigma_0 = rint(256 x), sigma_1 = rint(256 (256 x - sigma_0)), ...
-// with |sigma_t| <= 128, so that
-//   X_kj = f_j sum_t sigma_t,kj 256^-(t+1)   (error below f_j 2^-33).
+// Exact-product scheme for X^T X on bf16 MFMA.  The rows are cut into
+// groups G of kFoldRows; in each group, column j of X = W^1/2 J S is scaled
+// by f_Gj = 2^(E + 1), where max_{k in G} |X_kj| < 2^E, so x = X_kj / f_Gj
+// lies in (-1/2, 1/2), and cut into kSlices integer slices:
+// sigma_0 = rint(256 x), sigma_1 = rint(256 (256 x - sigma_0)), ... with
+// |sigma_t| <= 128, so that
+//   X_kj = f_Gj sum_t sigma_t,kj 256^-(t+1)   (error below f_Gj 2^-33).
 // Every sigma is an integer of at most 8 bits (exact in bf16) and every
-// product of two has at most 14 bits, so MFMA sums of up to 1024 products
-// of one order are exact in FP32.  The normal matrix
-//   (X^T X)_ij = f_i f_j sum_o 256^-(o+2) sum_{s+t=o} sum_k sigma_s,ki sigma_t,kj
+// product of two has at most 14 bits, so the MFMA sums of one order over
+// one group's 1024 rows are exact in FP32.  The normal matrix
+//   (X^T X)_ij = sum_G f_Gi f_Gj sum_o 256^-(o+2) sum_{s+t=o} sum_{k in G}
+//                sigma_s,ki sigma_t,kj
 // keeps the orders o = s + t <= 3 (10 slice pairs); the dropped orders are
-// below 2^-31 of f_i f_j n.  Slices are stored k-packed: slice t, rows
-// 8 g .. 8 g + 7 of column j are the 8 consecutive bf16 at
-// ((t * n_groups + g) * m + j) * 8, one 16-byte load per MFMA operand.
+// below 2^-31 of sum_G f_Gi f_Gj 1024.  A scale per group rather than per
+// column keeps the relative precision of columns whose entries fall off
+// with distance (J ~ 1/r): far rows are sliced against their own group's
+// largest entry.  Slices are stored k-packed: slice t, rows 8 g .. 8 g + 7
+// of column j are the 8 consecutive bf16 at ((t * n_groups + g) * m + j) * 8,
+// one 16-byte load per MFMA operand.
 constexpr int kSlices = 4;
 constexpr int kFoldChunks = 32;  // 32 chunks of 32 rows: order-0 sums stay exact
+constexpr int kFoldRows = 32 * kFoldChunks;
 
-// f_j = 2^(E_j + 1) with max_k sqrt(w_k) J_kj / s_j < 2^E_j (1 if all zero).
+// f[G][j] = 2^(E + 1) with max_{k in group G} sqrt(w_k) J_kj / s_j < 2^E
+// (1 if all zero), for the groups G of kFoldRows rows.
 __global__ __launch_bounds__(256) void colscale_pow2_kernel(
     const double *__restrict__ e, const double *__restrict__ n, const double *__restrict__ u,
     const double *__restrict__ w, int64_t n_data, const double *__restrict__ pe,
     const double *__restrict__ pn, const double *__restrict__ pu,
     const double *__restrict__ scale, int64_t n_src, double *__restrict__ f) {
+    static_assert(kFoldRows % 256 == 0, "a fold group is a whole number of tiles");
     __shared__ double4_t tile[256];
     const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
     const int64_t jj = j < n_src ? j : n_src - 1;
     const double se = pe[jj], sn = pn[jj], su = pu[jj];
+    const double inv_s = 1.0 / scale[jj];
+    auto put = [&](int64_t G, double mx) {
+        if (j >= n_src) return;
+        const double cmax = mx * inv_s;
+        int ex = 0;
+        (void)frexp(cmax, &ex);  // cmax < 2^ex
+        f[G * n_src + j] = cmax > 0.0 ? ldexp(1.0, ex + 1) : 1.0;
+    };
     double mx = 0.0;
     for (int64_t t0 = 0; t0 < n_data; t0 += 256) {
         const int cnt = (int)(n_data - t0 < 256 ? n_data - t0 : 256);
@@ -760,16 +776,15 @@ __global__ __launch_bounds__(256) void colscale_pow2_kernel(
             const double4_t o = tile[q];
             mx = fmax(mx, o.w * inv_dist(o.x, o.y, o.z, se, sn, su));
         }
-    }
-    if (j < n_src) {
-        const double cmax = mx / scale[j];
-        int ex = 0;
-        (void)frexp(cmax, &ex);  // cmax < 2^ex
-        f[j] = cmax > 0.0 ? ldexp(1.0, ex + 1) : 1.0;
+        if ((t0 + 256) % kFoldRows == 0 || t0 + 256 >= n_data) {  // end of a group
+            put(t0 / kFoldRows, mx);
+            mx = 0.0;
+        }
     }
 }
 
-// The kSlices slices of X / f in the k-packed layout (rows >= n_data are 0).
+// The kSlices slices of X / f (f per group of kFoldRows rows) in the
+// k-packed layout (rows >= n_data are 0).
 __global__ __launch_bounds__(256) void slice_kernel(
     const double *__restrict__ e, const double *__restrict__ n, const double *__restrict__ u,
     const double *__restrict__ w, int64_t n_data, const double *__restrict__ pe,
@@ -779,8 +794,9 @@ __global__ __launch_bounds__(256) void slice_kernel(
     const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (j >= n_src) return;
     const double se = pe[j], sn = pn[j], su = pu[j];
-    const double inv = 1.0 / (scale[j] * f[j]);
+    const double inv_s = 1.0 / scale[j];
     for (int64_t g = blockIdx.y; g < n_groups; g += gridDim.y) {
+        const double inv = inv_s / f[(8 * g / kFoldRows) * n_src + j];  // f: a power of 2
         bf16x8_t out[kSlices];
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
@@ -804,8 +820,8 @@ __global__ __launch_bounds__(256) void slice_kernel(
 }
 
 // Orders O0..O1 of the split-precision normal matrix on the upper-triangle
-// tiles: C = beta C + f_r f_c sum_o 256^-(o+2) sum_{s+t=o} S_s^T S_t
-// (+ shift on the diagonal).  A workgroup of 8 wavefronts owns a kST x kST
+// tiles: C = beta C + sum_G f_Gr f_Gc sum_o 256^-(o+2) sum_{s+t=o} S_s^T S_t
+// (+ shift on the diagonal), f the (n_fold, m) group scales.  A workgroup of 8 wavefronts owns a kST x kST
 // tile (2 x 4 waves of 64 x 32, 4 x 2 MFMA 16x16x32 bf16 tiles each) and
 // one FP32 accumulator set per order, folded into FP64 every kFoldChunks
 // chunks of 32 rows.  Slices 0..O1 of both panels go through two LDS
@@ -899,17 +915,29 @@ __global__ __launch_bounds__(512) void gram_bf16_kernel(const bf16x8_t *__restri
             if (more) store(cur ^ 1);
             __syncthreads();
         }
+        // fold: the group's exact order sums, scaled by f_Gr f_Gc (powers of 2)
+        const double *fg = f + (int64_t)(g0 / kFoldChunks) * m;
+        double fc[2];
 #pragma unroll
-        for (int o = 0; o < kNO; ++o) {
-            const double sc = ldexp(1.0, -8 * (O0 + o + 2));
-#pragma unroll
-            for (int ti = 0; ti < 4; ++ti)
-#pragma unroll
-                for (int tj = 0; tj < 2; ++tj)
-#pragma unroll
-                    for (int v = 0; v < 4; ++v)
-                        dacc[ti][tj][v] = fma((double)acc[o][ti][tj][v], sc, dacc[ti][tj][v]);
+        for (int tj = 0; tj < 2; ++tj) {
+            const int64_t c = c0 + 32 * wc + 16 * tj + (lane & 15);
+            fc[tj] = fg[c < m ? c : m - 1];
         }
+#pragma unroll
+        for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int64_t r = r0 + 64 * wr + 16 * ti + 4 * (lane >> 4) + v;
+                const double fr = fg[r < m ? r : m - 1];
+#pragma unroll
+                for (int tj = 0; tj < 2; ++tj) {
+                    double sum = 0.0;
+#pragma unroll
+                    for (int o = 0; o < kNO; ++o)
+                        sum = fma((double)acc[o][ti][tj][v], ldexp(1.0, -8 * (O0 + o + 2)), sum);
+                    dacc[ti][tj][v] = fma(sum, fr * fc[tj], dacc[ti][tj][v]);
+                }
+            }
     }
     double old[4][2][4];
     if (beta != 0.0) {
@@ -935,7 +963,7 @@ __global__ __launch_bounds__(512) void gram_bf16_kernel(const bf16x8_t *__restri
                 const int64_t r = r0 + 64 * wr + 16 * ti + 4 * (lane >> 4) + v;
                 const int64_t c = c0 + 32 * wc + 16 * tj + (lane & 15);
                 if (r < m && c < m && r <= c) {
-                    double val = dacc[ti][tj][v] * f[r] * f[c];
+                    double val = dacc[ti][tj][v];
                     if (r == c) val += shift;
                     C[r * ldc + c] = beta == 0.0 ? val : old[ti][tj][v] + val;
                 }
@@ -1180,7 +1208,8 @@ int hmx_eqs_fit_workspace_bytes(int64_t n_data, int64_t n_src, int precision_mod
     const size_t n_pad = (n + 31) / 32 * 32;
     size_t x_bytes = n * m * sizeof(double);                        // X = W^1/2 J S
     if (precision_mode == HMX_FIT_FP32_MFMA) x_bytes = n * m * sizeof(float);
-    if (precision_mode == HMX_FIT_BF16_SPLIT) x_bytes = kSlices * n_pad * m * 2;  // slices
+    if (precision_mode == HMX_FIT_BF16_SPLIT)  // slices and the group scales
+        x_bytes = kSlices * n_pad * m * 2 + (n_pad + kFoldRows - 1) / kFoldRows * m * sizeof(double);
     const size_t copies = precision_mode == HMX_FIT_FP64 ? 1 : 2;
     *bytes = copies * m * m * sizeof(double)             // A, then its factor U (+ a copy of A)
              + x_bytes + 2 * nblk * kNB * kNB * sizeof(double)  // inverses of diagonal blocks
@@ -1237,9 +1266,13 @@ int hmx_eqs_fit(const double *easting, const double *northing, const double *upw
         ws += (size_t)m * m * sizeof(double);
     }
     void *X = ws;
-    if (split)
+    double *fgroup = nullptr;  // split: (n_fold, m) power-of-2 column scales per row group
+    const int64_t n_fold = (n_groups * 8 + kFoldRows - 1) / kFoldRows;
+    if (split) {
         ws += (size_t)kSlices * n_groups * 8 * m * 2;
-    else
+        fgroup = reinterpret_cast<double *>(ws);
+        ws += (size_t)n_fold * m * sizeof(double);
+    } else
         ws += (size_t)nd * m * (mixed ? sizeof(float) : sizeof(double));
     f.Dinv = reinterpret_cast<double *>(ws);
     f.DinvT = f.Dinv + nblk * kNB * kNB;
@@ -1265,11 +1298,12 @@ int hmx_eqs_fit(const double *easting, const double *northing, const double *upw
     // precision.
     if (split) {
         hipLaunchKernelGGL(colscale_pow2_kernel, dim3(gm), dim3(256), 0, s, easting, northing,
-                           upward, weights, nd, p_easting, p_northing, p_upward, f.scale, m, pow2);
+                           upward, weights, nd, p_easting, p_northing, p_upward, f.scale, m,
+                           fgroup);
         HMX_LAUNCHED("colscale_pow2_kernel");
         const unsigned gg = (unsigned)(n_groups < 4096 ? n_groups : 4096);
         hipLaunchKernelGGL(slice_kernel, dim3(gm, gg), dim3(256), 0, s, easting, northing, upward,
-                           weights, nd, p_easting, p_northing, p_upward, f.scale, pow2, m,
+                           weights, nd, p_easting, p_northing, p_upward, f.scale, fgroup, m,
                            n_groups, static_cast<bf16x8_t *>(X));
         HMX_LAUNCHED("slice_kernel");
     } else if (mixed) {
@@ -1316,10 +1350,10 @@ int hmx_eqs_fit(const double *easting, const double *northing, const double *upw
                 const unsigned blocks = (unsigned)(tiles * (tiles + 1) / 2);
                 const bf16x8_t *Sl = static_cast<const bf16x8_t *>(X);
                 hipLaunchKernelGGL((gram_bf16_kernel<0, 1>), dim3(blocks), dim3(512), 0, s, Sl,
-                                   n_groups, m, pow2, f.A, m, 0.0, damping, tiles);
+                                   n_groups, m, fgroup, f.A, m, 0.0, damping, tiles);
                 HMX_LAUNCHED("gram_bf16_kernel<0,1>");
                 hipLaunchKernelGGL((gram_bf16_kernel<2, 3>), dim3(blocks), dim3(512), 0, s, Sl,
-                                   n_groups, m, pow2, f.A, m, 1.0, 0.0, tiles);
+                                   n_groups, m, fgroup, f.A, m, 1.0, 0.0, tiles);
                 HMX_LAUNCHED("gram_bf16_kernel<2,3>");
             } else {
                 st = launch_syrk<float>(static_cast<const float *>(X), m, nd, m, f.A, m, 0.0, 1.0,
