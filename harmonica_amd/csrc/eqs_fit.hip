@@ -327,6 +327,50 @@ __device__ __forceinline__ void tile_k_loop(const Panel<T> &A, const Panel<T> &B
             }
 }
 
+// Upper-triangle tiles in an XCD-aware order, for the bf16 split-product
+// kernel, whose inputs stream from beyond L2 (the FP64 GEMM, bound by its
+// MFMA rate, runs 3% slower in this order and keeps the row-major one).
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
+// L2, and the 32 that one XCD runs at a time are consecutive in that XCD's
+// share of the ids; they take one super-block of 4 row tiles x 8 column
+// tiles, so the panels they stream through K are shared in that L2 (12
+// panels per super-block instead of 33 for a row of 32 tiles).
+// Super-blocks that reach the upper triangle run row-major; tiles of a
+// super-block below the diagonal or past the edge get no work (false).
+constexpr int kXcds = 8, kSbRows = 4, kSbCols = 8;
+
+__host__ __device__ inline int64_t superblock_first_col(int64_t sr) {
+    return sr * kSbRows / kSbCols;  // first super-column that reaches the diagonal
+}
+
+__device__ __forceinline__ bool upper_tile_xcd(int64_t id, int64_t tiles, int64_t &rb,
+                                               int64_t &cb) {
+    const int64_t x = id % kXcds, q = id / kXcds;
+    int64_t S = kXcds * (q / (kSbRows * kSbCols)) + x;
+    const int64_t p = q % (kSbRows * kSbCols);
+    const int64_t C = (tiles + kSbCols - 1) / kSbCols;
+    int64_t sr = 0;
+    for (;;) {
+        const int64_t cnt = C - superblock_first_col(sr);
+        if (cnt <= 0) return false;  // past the last super-block
+        if (S < cnt) break;
+        S -= cnt;
+        ++sr;
+    }
+    rb = sr * kSbRows + p / kSbCols;
+    cb = (superblock_first_col(sr) + S) * kSbCols + p % kSbCols;
+    return rb < tiles && cb < tiles && cb >= rb;
+}
+
+// Workgroups for upper_tile_xcd over `tiles` x `tiles` tiles.
+inline unsigned upper_tile_xcd_blocks(int64_t tiles) {
+    const int64_t R = (tiles + kSbRows - 1) / kSbRows, C = (tiles + kSbCols - 1) / kSbCols;
+    int64_t n = 0;
+    for (int64_t sr = 0; sr < R; ++sr) n += C - superblock_first_col(sr);
+    n = (n + kXcds - 1) / kXcds * kXcds;
+    return (unsigned)(n * kSbRows * kSbCols);
+}
+
 // C = beta * C + sign * A^T B + shift * I on the selected tiles, with A
 // (K x M) and B (K x N) row-major panels and C (M x N) row-major (ldc).
 // UPPER: M == N and only the upper triangle (r <= c) is computed (the
@@ -348,15 +392,15 @@ __global__ __launch_bounds__(256) void gemm_tn_kernel(Panel<T> A, Panel<T> B, in
                                                       int64_t tiles_n) {
     __shared__ T lds[2][2][kKC * kPad];
     int64_t rb, cb;
-    if (UPPER) {
-        int64_t b = blockIdx.x, rem = tiles_n;
+    if (UPPER) {  // row-major over the upper triangle (FP64 MFMA is compute-bound)
+        int64_t bq = blockIdx.x, rem = tiles_n;
         rb = 0;
-        while (b >= rem) {
-            b -= rem;
+        while (bq >= rem) {
+            bq -= rem;
             ++rb;
             --rem;
         }
-        cb = rb + b;
+        cb = rb + bq;
     } else {
         rb = blockIdx.x / tiles_n;
         cb = blockIdx.x - rb * tiles_n;
@@ -836,13 +880,8 @@ __global__ __launch_bounds__(512) void gram_bf16_kernel(const bf16x8_t *__restri
     constexpr int kNO = O1 - O0 + 1;  // orders accumulated here
     constexpr int kNS = O1 + 1;       // slices needed on each side
     __shared__ bf16x8_t lds[2][2][kNS][4][kST];
-    int64_t b = blockIdx.x, rem = tiles_n, rb = 0;
-    while (b >= rem) {
-        b -= rem;
-        ++rb;
-        --rem;
-    }
-    const int64_t cb = rb + b;
+    int64_t rb, cb;
+    if (!upper_tile_xcd(blockIdx.x, tiles_n, rb, cb)) return;
     const int64_t r0 = rb * kST, c0 = cb * kST;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wr = wave >> 2, wc = wave & 3;
@@ -1347,7 +1386,7 @@ int hmx_eqs_fit(const double *easting, const double *northing, const double *upw
         } else if (mixed) {
             if (split) {
                 const int64_t tiles = (m + kST - 1) / kST;
-                const unsigned blocks = (unsigned)(tiles * (tiles + 1) / 2);
+                const unsigned blocks = upper_tile_xcd_blocks(tiles);
                 const bf16x8_t *Sl = static_cast<const bf16x8_t *>(X);
                 hipLaunchKernelGGL((gram_bf16_kernel<0, 1>), dim3(blocks), dim3(512), 0, s, Sl,
                                    n_groups, m, fgroup, f.A, m, 0.0, damping, tiles);
