@@ -457,61 +457,62 @@ __global__ __launch_bounds__(1024) void potrf_diag_kernel(double *__restrict__ D
                                                           int *__restrict__ info) {
     constexpr int S = kNB + 1;
     __shared__ double L[kNB * S];
+    __shared__ double piv[kNB];
     __shared__ double col[kNB];
-    __shared__ int bad;
-    for (int idx = threadIdx.x; idx < nb * nb; idx += blockDim.x) {
-        const int r = idx / nb, c = idx - r * nb;
-        L[r * S + c] = D[r * ld + c];
-    }
-    if (threadIdx.x == 0) bad = 0;
+    const int tid = threadIdx.x, tx = tid & 31, ty = tid >> 5;  // 32 x 32 threads
+    for (int r = ty; r < nb; r += 32)
+        for (int c = tx; c < nb; c += 32) L[r * S + c] = D[r * ld + c];
     __syncthreads();
+    // Right-looking factorisation, one barrier per column: every thread
+    // reads the pivot d (the same value, so the test is uniform) and updates
+    // its part of the trailing upper triangle by L[k][r] L[k][c] / d; row k
+    // itself is read only in step k and is scaled by its pivot at the end.
     for (int k = 0; k < nb; ++k) {
         const double d = L[k * S + k];
         if (!(d > 0.0)) {
-            if (threadIdx.x == 0) {
-                bad = 1;
-                atomicCAS(info, 0, (int)(col0 + k + 1));
-            }
+            if (tid == 0) atomicCAS(info, 0, (int)(col0 + k + 1));
+            return;
         }
-        __syncthreads();
-        if (bad) return;
-        const double piv = sqrt(d);
-        const double inv = 1.0 / piv;
-        for (int c = k + 1 + threadIdx.x; c < nb; c += blockDim.x) L[k * S + c] *= inv;
-        __syncthreads();
-        if (threadIdx.x == 0) L[k * S + k] = piv;
-        const int rem = nb - k - 1;
-        for (int idx = threadIdx.x; idx < rem * rem; idx += blockDim.x) {
-            const int r = k + 1 + idx / rem, c = k + 1 + idx % rem;
-            if (r <= c) L[r * S + c] = fma(-L[k * S + r], L[k * S + c], L[r * S + c]);
+        if (tid == 0) piv[k] = sqrt(d);
+        const double invd = 1.0 / d;
+        for (int r = k + 1 + ty; r < nb; r += 32) {
+            const double lr = L[k * S + r] * invd;
+            for (int c = r + tx; c < nb; c += 32) L[r * S + c] = fma(-lr, L[k * S + c], L[r * S + c]);
         }
         __syncthreads();
     }
-    for (int idx = threadIdx.x; idx < nb * nb; idx += blockDim.x) {
-        const int r = idx / nb, c = idx - r * nb;
-        D[r * ld + c] = r <= c ? L[r * S + c] : 0.0;
-    }
-    // In-place inverse of the upper triangular factor, column by column:
-    // X[j][j] = 1 / U[j][j], X[i][j] = -X[j][j] sum_{k=i}^{j-1} X[i][k] U[k][j].
-    for (int j = 0; j < nb; ++j) {
-        __syncthreads();
-        for (int i = threadIdx.x; i <= j; i += blockDim.x) col[i] = L[i * S + j];
-        __syncthreads();
-        const double djj = 1.0 / col[j];
-        for (int i = threadIdx.x; i < j; i += blockDim.x) {
-            double acc = 0.0;
-            for (int k = i; k < j; ++k) acc = fma(L[i * S + k], col[k], acc);
-            L[i * S + j] = -djj * acc;
-        }
-        if (threadIdx.x == 0) L[j * S + j] = djj;
+    for (int r = ty; r < nb; r += 32) {
+        const double inv = 1.0 / piv[r];
+        for (int c = r + tx; c < nb; c += 32) L[r * S + c] = c == r ? piv[r] : L[r * S + c] * inv;
     }
     __syncthreads();
-    for (int idx = threadIdx.x; idx < kNB * kNB; idx += blockDim.x) {
-        const int r = idx / kNB, c = idx - r * kNB;
-        const double v = (r < nb && c < nb && r <= c) ? L[r * S + c] : 0.0;
-        Dinv[r * kNB + c] = v;
-        DinvT[c * kNB + r] = v;
+    for (int r = ty; r < nb; r += 32)
+        for (int c = tx; c < nb; c += 32) D[r * ld + c] = r <= c ? L[r * S + c] : 0.0;
+    // In-place inverse of the upper triangular factor, column by column:
+    // X[j][j] = 1 / U[j][j], X[i][j] = -X[j][j] sum_{k=i}^{j-1} X[i][k] U[k][j];
+    // each sum is split over the 8 adjacent lanes of its row and combined
+    // with shuffles.
+    const int ir = tid >> 3, part = tid & 7;
+    for (int j = 0; j < nb; ++j) {
+        if (tid < nb) col[tid] = tid <= j ? L[tid * S + j] : 0.0;
+        __syncthreads();
+        const double djj = 1.0 / col[j];
+        double acc = 0.0;
+        if (ir < j)
+            for (int k = ir + part; k < j; k += 8) acc = fma(L[ir * S + k], col[k], acc);
+        acc += __shfl_xor(acc, 1, 8);
+        acc += __shfl_xor(acc, 2, 8);
+        acc += __shfl_xor(acc, 4, 8);
+        if (part == 0 && ir < j) L[ir * S + j] = -djj * acc;
+        if (tid == 0) L[j * S + j] = djj;
+        __syncthreads();
     }
+    for (int r = ty; r < kNB; r += 32)
+        for (int c = tx; c < kNB; c += 32) {
+            const double v = (r < nb && c < nb && r <= c) ? L[r * S + c] : 0.0;
+            Dinv[r * kNB + c] = v;
+            DinvT[c * kNB + r] = v;
+        }
 }
 
 // ------------------------------------------------------------------ solves
