@@ -196,7 +196,7 @@ def bench_batched(args):
             "max_rel_diff_vs_single_model": worst}
 
 
-def bench_eqs(args):
+def bench_eqs(args, modes=None):
     """Equivalent-source fit of 60,000 scattered g_z data (3 prisms + 1 mGal
     noise), damping 1e-3, depth 4500 m: FP64 fit vs mixed-precision fit,
     predictions on a 512 x 512 grid at 1000 m."""
@@ -209,7 +209,8 @@ def bench_eqs(args):
     out = {"n_data": coords[0].size}
     preds = {}
     small = tuple(c[:2000] for c in coords)
-    modes = ("float64", "mixed", "mixed-fp32") if args.eqs_all else ("float64", "mixed")
+    if modes is None:
+        modes = ("float64", "mixed", "mixed-fp32") if args.eqs_all else ("float64", "mixed")
     for mode in modes:  # untimed warm-up of every code path
         hm.EquivalentSources(damping=w["damping"], depth=w["depth"], fit_precision=mode).fit(
             small, data[:2000])
@@ -220,8 +221,8 @@ def bench_eqs(args):
         preds[mode] = pred
         out[mode] = {"fit_s": dt, "predict_s": dtp, "info": eqs.fit_info_}
         log(f"eqs {mode}: fit {dt:.2f} s, predict {dtp:.3f} s, {eqs.fit_info_}")
-    ref = preds["float64"]
-    for mode in modes[1:]:
+    ref = preds.get("float64")
+    for mode in modes[1:] if ref is not None else ():
         out[mode]["max_rel_pred_diff_vs_fp64"] = float(np.abs(preds[mode] - ref).max() /
                                                        np.abs(ref).max())
         out[mode]["speedup_vs_fp64"] = out["float64"]["fit_s"] / out[mode]["fit_s"]
@@ -309,6 +310,8 @@ def main():
     ap.add_argument("--eqs-data", type=int, default=60000)
     ap.add_argument("--eqs-oracle", action="store_true", help="check the FP64 fit on a 20k subset")
     ap.add_argument("--inversion", action="store_true", help="also run the autograd inversion")
+    ap.add_argument("--eqs-only", choices=("float64", "mixed", "mixed-fp32"),
+                    help="only run the equivalent-source fit in this precision (for profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device-to-device (calibrates FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -326,6 +329,10 @@ def main():
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path's "
               "multi-GPU sharding is described in DESIGN.md", file=sys.stderr)
         sys.exit(2)
+
+    if args.eqs_only:
+        print(json.dumps({"eqs_fit": bench_eqs(args, (args.eqs_only,))}))
+        return
 
     import torch
 
