@@ -54,6 +54,15 @@ FLOPS_PER_PAIR_ALL = 953.6
 VALU_INSTR_PER_PAIR_ALL = 825.1
 TRANS_PER_PAIR_ALL = 32.0
 SIMD_CYCLES_PER_S = 1024 * 2.4e9
+# Same counters for prism_magnetic_kernel<b_e|b_n|b_u> and <b_u> on the 1M
+# terrain prisms with their magnetisation (lean tensor forms: one logarithm
+# per off-diagonal and one arctangent per diagonal term, K_uu by Laplace).
+FLOPS_PER_PAIR_B = 474.5
+VALU_INSTR_PER_PAIR_B = 424.1
+TRANS_PER_PAIR_B = 16.12
+FLOPS_PER_PAIR_BU = 314.4
+VALU_INSTR_PER_PAIR_BU = 276.8
+TRANS_PER_PAIR_BU = 12.53
 
 
 def valu_issue_frac(pair_rate, valu, trans):
@@ -165,6 +174,39 @@ def bench_point(args):
         _, dt = _sync_time(lambda: point_fields_device(E, N, U, *P, M, fields))
         out[name] = {"s": dt, "pair_evals_per_s": pairs / dt}
         log(f"point {name}: {dt:.4f} s, {pairs / dt:.4e} pair-evals/s")
+    return out
+
+
+def bench_magnetic(args):
+    """prism_magnetic field "b" (three components in one launch) of the 1M
+    terrain prisms with induced + remanent magnetisation at the 256 x 256
+    grid at 2500 m (hmx_prism_magnetic, inputs in HBM)."""
+    from harmonica_amd._forward import to_device
+    from harmonica_amd._magnetic import magnetic_fields_device
+    from harmonica_amd.workloads import layer_to_prisms, terrain_layer_large, terrain_magnetization
+
+    w = terrain_layer_large(n_prisms_side=args.prism_side, n_obs_side=args.obs_side)
+    prisms = layer_to_prisms(w["easting"], w["northing"], w["surface"], w["reference"])
+    mag = terrain_magnetization(n_prisms_side=args.prism_side)
+    E, N, U = (to_device(c.ravel()) for c in w["coordinates"])
+    P = to_device(prisms)
+    M = [to_device(m) for m in mag]
+    pairs = E.numel() * P.shape[0]
+    out = {}
+    counts = {"b": (FLOPS_PER_PAIR_B, VALU_INSTR_PER_PAIR_B, TRANS_PER_PAIR_B),
+              "b_u": (FLOPS_PER_PAIR_BU, VALU_INSTR_PER_PAIR_BU, TRANS_PER_PAIR_BU)}
+    full_size = args.prism_side == 1000 and args.obs_side == 256
+    for name, comps in (("b", ("b_e", "b_n", "b_u")), ("b_u", ("b_u",))):
+        magnetic_fields_device(E, N, U, P, *M, comps)
+        _, dt = _sync_time(lambda: magnetic_fields_device(E, N, U, P, *M, comps))
+        rate = pairs / dt
+        out[name] = {"s": dt, "pair_evals_per_s": rate}
+        if full_size:  # the counters were taken on this workload
+            flops, valu, trans = counts[name]
+            out[name].update({"fp64_tflops": rate * flops / 1e12,
+                              "fp64_peak_frac": rate * flops / FP64_PEAK,
+                              "valu_issue_frac": valu_issue_frac(rate, valu, trans)})
+        log(f"magnetic {name}: {dt:.4f} s, {rate:.4e} pair-evals/s")
     return out
 
 
@@ -302,6 +344,8 @@ def main():
     ap.add_argument("--prism-side", type=int, default=1000)
     ap.add_argument("--layer", action="store_true", help="also time the layer accessor path")
     ap.add_argument("--point", action="store_true", help="also time point_gravity (1M masses)")
+    ap.add_argument("--magnetic", action="store_true",
+                    help="also time prism_magnetic (field b) on the 1M terrain prisms")
     ap.add_argument("--batched", action="store_true", help="also run the 512-model batch")
     ap.add_argument("--batched-models", type=int, default=512)
     ap.add_argument("--no-eqs", action="store_true",
@@ -411,6 +455,8 @@ def main():
         extra["layer_path"] = bench_layer_fused(args)
     if args.point:
         extra["point_masses"] = bench_point(args)
+    if args.magnetic:
+        extra["magnetic"] = bench_magnetic(args)
     if args.batched:
         extra["batched"] = bench_batched(args)
     if not args.no_eqs:

@@ -8,6 +8,7 @@ Drop-in functions with the reference's Python API (fatiando/harmonica):
 * ``prism_layer`` and its ``.prism_layer`` accessor (``.gravity``),
 * ``point_gravity`` (Cartesian and spherical),
 * ``EquivalentSources`` (jacobian / predict / fit / grid / profile),
+* ``prism_magnetic`` (b_e, b_n, b_u or the vector ``b``),
 
 computed by hand-written HIP kernels in libhmx.so (C-ABI in include/hmx.h),
 called through ctypes, with PyTorch-ROCm used only for device buffers and
@@ -18,6 +19,7 @@ functions raise.
 from ._eqs import EquivalentSources
 from ._forward import point_gravity, prism_gravity
 from ._layer import DatasetAccessorPrismLayer, prism_layer
+from ._magnetic import prism_magnetic
 from .constants import GRAVITATIONAL_CONST
 
 __all__ = [
@@ -27,6 +29,7 @@ __all__ = [
     "point_gravity",
     "prism_gravity",
     "prism_layer",
+    "prism_magnetic",
 ]
 
 __version__ = "0.1.0"

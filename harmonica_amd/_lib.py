@@ -52,6 +52,9 @@ SIGNATURES = {
     "hmx_prism_singular_scan": (_int, [_vp, _vp, _vp, _i64, _vp, _i64, _u32, _vp, _vp]),
     "hmx_prism_layer_gravity": (_int, [_vp, _vp, _vp, _i64, _vp, _i64, _vp, _i64, _vp, _vp, _vp,
                                        _dbl, _u32, _vp, _vp, _vp, ctypes.c_size_t, _vp]),
+    "hmx_prism_magnetic_workspace_bytes": (_int, [_i64, _i64, _u32, _szp]),
+    "hmx_prism_magnetic": (_int, [_vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _i64, _u32, _vp, _vp,
+                                  ctypes.c_size_t, _vp]),
     "hmx_point_gravity": (_int, [_vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _i64, _int, _u32, _vp,
                                  _vp, ctypes.c_size_t, _vp]),
     "hmx_eqs_jacobian": (_int, [_vp, _vp, _vp, _i64, _vp, _vp, _vp, _i64, _int, _vp, _i64, _vp]),

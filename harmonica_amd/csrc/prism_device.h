@@ -115,16 +115,26 @@ __device__ __forceinline__ double log_quad(double a0, double a1, double r00, dou
 // pair is (0, D >= 0)).
 // kZeroSafe: z, x_i or y_j may be 0 (then the angle pair may be (0, 0) and
 // the result is 0); without it none of them is.
-template <bool kWrap, bool kZeroSafe = true>
-__device__ __forceinline__ double face_angle(double x0, double x1, double y0, double y1, double z,
-                                             double r00, double r01, double r10, double r11,
-                                             bool inside, const dm::Tables &tab) {
+// The angle pair (N, D) of face_angle below: the face's signed solid angle
+// is atan2(N, D) modulo 2 pi.
+__device__ __forceinline__ void face_nd(double x0, double x1, double y0, double y1, double z,
+                                        double r00, double r01, double r10, double r11,
+                                        double &N, double &D) {
     const double n0 = y0 * z * fma(x0, r10, -x1 * r00);
     const double d0 = fma(z * z, r00 * r10, x0 * x1 * (y0 * y0));
     const double n1 = y1 * z * fma(x0, r11, -x1 * r01);
     const double d1 = fma(z * z, r01 * r11, x0 * x1 * (y1 * y1));
-    const double a =
-        dm::atan2_full<kZeroSafe>(fma(n0, d1, -n1 * d0), fma(d0, d1, n0 * n1), tab);
+    N = fma(n0, d1, -n1 * d0);
+    D = fma(d0, d1, n0 * n1);
+}
+
+template <bool kWrap, bool kZeroSafe = true>
+__device__ __forceinline__ double face_angle(double x0, double x1, double y0, double y1, double z,
+                                             double r00, double r01, double r10, double r11,
+                                             bool inside, const dm::Tables &tab) {
+    double N, D;
+    face_nd(x0, x1, y0, y1, z, r00, r01, r10, r11, N, D);
+    const double a = dm::atan2_full<kZeroSafe>(N, D, tab);
     if constexpr (kWrap) {
         constexpr double kTwoPi = 6.283185307179586232;
         const bool wrap = inside && a * z < 0.0;
@@ -394,6 +404,86 @@ __device__ __forceinline__ void prism_fields(double x0, double x1, double y0, do
     }
 }
 
+// The second-derivative vertex sums K[f - 4] (f = 4..9: ee, nn, uu, en,
+// eu, nu) of the gravity field bits in TM, for a point that lies between the
+// two planes of no axis and on no face plane of the prism (relative bounds
+// as prism_fields).  There every logarithm of one axis takes the same branch
+// of log_diff and no face is seen from inside, so the sums regroup further:
+//  * K_en = sum (-1)^(i+j+k) L_z = +-ln(prod_even t / prod_odd t), t = r + |z|
+//    at the vertices of even and odd i + j + k, the sign that of log_diff for
+//    the z-axis: one logarithm instead of four.  K_eu (along y) and K_nu
+//    (along x) likewise.
+//  * K_ee = -(O_x(x0) - O_x(x1)): the two face angles have the sign of
+//    x0 (= that of x1) and magnitude below pi, so their difference lies in
+//    (-pi, pi) and is one two-argument angle formed from the faces' (N, D)
+//    pairs: one arctangent instead of two.  K_nn, K_uu likewise; with all
+//    three, K_uu = -(K_ee + K_nn) (Laplace's equation outside the prism).
+template <unsigned TM>
+__device__ __forceinline__ void tensor_lean(double x0, double x1, double y0, double y1, double z0,
+                                            double z1, double (&K)[6], const dm::Tables &tab) {
+    using F = FieldSet<TM>;
+    const double xs[2] = {x0 * x0, x1 * x1}, ys[2] = {y0 * y0, y1 * y1};
+    const double zs[2] = {z0 * z0, z1 * z1};
+    const double X[2] = {x0, x1}, Y[2] = {y0, y1}, Z[2] = {z0, z1};
+    double r[2][2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int k = 0; k < 2; ++k) r[i][j][k] = vertex_r(xs[i] + ys[j], z_tiny(zs[k]));
+    // +-ln(prod over even vertices of (r + |a|) / prod over odd ones)
+    auto off = [&](auto coord, double a0) {
+        double num = 1.0, den = 1.0;
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+#pragma unroll
+                for (int k = 0; k < 2; ++k) {
+                    const double t = r[i][j][k] + fabs(coord(i, j, k));
+                    if ((i + j + k) & 1)
+                        den *= t;
+                    else
+                        num *= t;
+                }
+        return dm::flip_sign(dm::log_pos(dm::div(num, den), tab), dm::sign_mask(a0 < 0.0));
+    };
+    if constexpr (F::en) K[3] = off([&](int, int, int k) { return Z[k]; }, z0);
+    if constexpr (F::eu) K[4] = off([&](int, int j, int) { return Y[j]; }, y0);
+    if constexpr (F::nu) K[5] = off([&](int i, int, int) { return X[i]; }, x0);
+    // -(A(face 0) - A(face 1)) from the two faces' angle pairs
+    auto diag = [&](double N0, double D0, double N1, double D1) {
+        return dm::atan2_full<false>(fma(N1, D0, -N0 * D1), fma(D0, D1, N0 * N1), tab);
+    };
+    constexpr bool kLaplace = F::ee && F::nn && F::uu;
+    if constexpr (F::ee) {
+        double N0, D0, N1, D1;
+        face_nd(y0, y1, z0, z1, x0, r[0][0][0], r[0][0][1], r[0][1][0], r[0][1][1], N0, D0);
+        face_nd(y0, y1, z0, z1, x1, r[1][0][0], r[1][0][1], r[1][1][0], r[1][1][1], N1, D1);
+        K[0] = diag(N0, D0, N1, D1);
+    }
+    if constexpr (F::nn) {
+        double N0, D0, N1, D1;
+        face_nd(z0, z1, x0, x1, y0, r[0][0][0], r[1][0][0], r[0][0][1], r[1][0][1], N0, D0);
+        face_nd(z0, z1, x0, x1, y1, r[0][1][0], r[1][1][0], r[0][1][1], r[1][1][1], N1, D1);
+        K[1] = diag(N0, D0, N1, D1);
+    }
+    if constexpr (F::uu) {
+        if constexpr (kLaplace) {
+            K[2] = -(K[0] + K[1]);
+        } else {
+            double N0, D0, N1, D1;
+            face_nd(x0, x1, y0, y1, z0, r[0][0][0], r[0][1][0], r[1][0][0], r[1][1][0], N0, D0);
+            face_nd(x0, x1, y0, y1, z1, r[0][0][1], r[0][1][1], r[1][0][1], r[1][1][1], N1, D1);
+            K[2] = diag(N0, D0, N1, D1);
+        }
+    }
+    (void)X;
+    (void)Y;
+    (void)Z;
+}
+
 // Whether the point (e, n, u) needs the general instantiation for this
 // prism: for g_u, g_e, g_n alone when it lies between the two planes of one
 // of the two axes the logarithms run along; for other field sets when it
@@ -430,6 +520,12 @@ __device__ __forceinline__ void prism_pair_path(double e, double n, double u, do
     } else if constexpr (M == 0x4u) {  // g_n alone: K_n is K_u with (x, y, z) -> (z, x, y)
         acc[0] += grho * gu_sum<false, kGeneral>(T - u, B - u, E - e, W - e, N - n, S - n, tab,
                                                  nullptr);
+    } else if constexpr ((M & 0xfu) == 0 && !kGeneral) {  // tensor terms only
+        double K[6];
+        tensor_lean<M>(E - e, W - e, N - n, S - n, T - u, B - u, K, tab);
+#pragma unroll
+        for (int f = 4; f < 10; ++f)
+            if ((M >> f) & 1u) acc[F::slot(f)] += grho * K[f - 4];
     } else {
         double vs[F::count];
         prism_fields<M, kGeneral>(E - e, W - e, N - n, S - n, T - u, B - u, vs, sflags, tab);

@@ -12,6 +12,10 @@ reference's tests.
   ten fields on a 256 x 256 grid at 2500 m (6.55e10 pairs).
 * batched_models       512 independent prism models with log-uniform prism
   (1e3-2e5) and observation (1e3-5e4) counts.
+* terrain_magnetization  magnetisation of the terrain_layer_large prisms:
+  induced by a 50,000 nT field (inclination -30, declination 12 degrees)
+  with fractal susceptibility in [0.001, 0.05] SI, plus N(0, 0.2 A/m)
+  remanence per component.
 * eqs_problem          60,000 scattered g_z data from 3 prisms + 1 mGal noise.
 * inversion_layer      200 x 200 basement layer and a 128 x 128 grid at 500 m.
 """
@@ -78,6 +82,17 @@ def terrain_layer_large(seed=2, n_prisms_side=1000, n_obs_side=256):
         "density": density,
         "coordinates": _grid(0, 1e5, 0, 1e5, (n_obs_side, n_obs_side), 2500.0),
     }
+
+
+def terrain_magnetization(n_prisms_side=1000, seed=5):
+    """(east, north, up) magnetisation in A/m of the terrain_layer_large
+    prisms, in layer_to_prisms order (northing-major ravel)."""
+    rng = np.random.default_rng(seed)
+    chi = fractal_surface(n_prisms_side, rng, 0.001, 0.05).ravel()
+    inc, dec = np.radians(-30.0), np.radians(12.0)
+    h0 = 50000e-9 / (4e-7 * np.pi)  # |B0| / mu_0 in A/m
+    unit = (np.cos(inc) * np.sin(dec), np.cos(inc) * np.cos(dec), -np.sin(inc))
+    return tuple(chi * h0 * d + rng.normal(0.0, 0.2, chi.size) for d in unit)
 
 
 def layer_to_prisms(easting, northing, surface, reference):

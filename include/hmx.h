@@ -53,6 +53,12 @@ extern "C" {
 #define HMX_G_NZ (1u << 9)
 #define HMX_ALL_FIELDS 0x3ffu
 
+/* Components of hmx_prism_magnetic, in the order of the reference's
+ * field "b" tuple (magnetic.py:195-199). */
+#define HMX_B_E (1u << 0)
+#define HMX_B_N (1u << 1)
+#define HMX_B_U (1u << 2)
+
 /* Coordinate systems of hmx_point_gravity (point.py:231-233). */
 #define HMX_CARTESIAN 0
 #define HMX_SPHERICAL 1
@@ -125,6 +131,28 @@ int hmx_prism_layer_gravity(const double *easting, const double *northing, const
                             double thickness_threshold, uint32_t field_mask, double *out,
                             uint32_t *singular_flags, void *workspace, size_t workspace_bytes,
                             void *stream);
+
+/* Bytes of workspace needed by hmx_prism_magnetic (as
+ * hmx_forward_workspace_bytes). */
+int hmx_prism_magnetic_workspace_bytes(int64_t n_obs, int64_t n_prisms, uint32_t component_mask,
+                                       size_t *bytes);
+
+/* Replaces _jit_prism_magnetic_field (src/harmonica/_forward/prisms/magnetic.py:275-338,
+ * field "b") and _jit_prism_magnetic_component (magnetic.py:341-400, one
+ * component) with choclo's per-pair prism magnetic functions, plus the nT
+ * conversion (magnetic.py:195-197, :271).
+ *   prisms : (n_prisms, 6) row-major [west, east, south, north, bottom, top]
+ *   magnetization_e/_n/_u : (n_prisms) magnetisation components [A/m]
+ *   component_mask : HMX_B_E | HMX_B_N | HMX_B_U (any non-empty subset)
+ *   out    : (popcount(component_mask), n_obs) [nT], overwritten
+ * A value is NaN where the point lies on an edge or vertex of a prism.
+ * Null prisms are NOT discarded here (the Python layer does it as
+ * _discard_null_prisms, magnetic.py:403-440, does). */
+int hmx_prism_magnetic(const double *easting, const double *northing, const double *upward,
+                       int64_t n_obs, const double *prisms, const double *magnetization_e,
+                       const double *magnetization_n, const double *magnetization_u,
+                       int64_t n_prisms, uint32_t component_mask, double *out, void *workspace,
+                       size_t workspace_bytes, void *stream);
 
 /* Replaces point_mass_cartesian / point_mass_spherical
  * (src/harmonica/_forward/point.py:357-454) with the choclo.point kernels

@@ -61,6 +61,8 @@ def lib():
         L.or_point_spherical.argtypes = [dp, dp, dp, i64, dp, dp, dp, dp, i64, ci, dp, ci]
         L.or_eqs_jacobian.argtypes = [dp, dp, dp, i64, dp, dp, dp, i64, dp, ci]
         L.or_eqs_predict.argtypes = [dp, dp, dp, i64, dp, dp, dp, dp, i64, dp, ci]
+        L.or_prism_magnetic.argtypes = [dp, dp, dp, i64, dp, dp, i64, ci, dp, ci]
+        L.or_prism_magnetic_pair.argtypes = [cd, cd, cd, dp, cd, cd, cd, dp]
         L.or_num_threads.restype = ci
         _lib = L
     return _lib
@@ -209,6 +211,41 @@ def eqs_fit_coefs(jacobian, data, weights=None, damping=None):
 
         sol = scipy.linalg.solve(A, b, assume_a="pos")
     return sol / scale
+
+
+MAGNETIC_FIELDS = ("b", "b_e", "b_n", "b_u")
+
+
+def prism_magnetic(coordinates, prisms, magnetization, field, n_threads=0):
+    """Oracle of ``harmonica.prism_magnetic`` (magnetic.py:28-134) without the
+    input checks: null prisms (zero volume or zero magnetisation) discarded
+    as magnetic.py:403-440 does, nT (magnetic.py:195-197,271), the tuple
+    (b_e, b_n, b_u) for ``field="b"``."""
+    shape, (e, n, u) = _coords(coordinates)
+    prisms = _f64(np.atleast_2d(prisms))
+    mag = np.stack([_f64(np.atleast_1d(m).ravel()) for m in magnetization])
+    w, ea, s_, no, b, t = (prisms[:, i] for i in range(6))
+    null = (w == ea) | (s_ == no) | (b == t) | np.all(mag == 0, axis=0)
+    prisms = _f64(prisms[~null])
+    mag = _f64(mag[:, ~null])
+    comp = 7 if field == "b" else 1 << ("b_e", "b_n", "b_u").index(field)
+    rows = 3 if field == "b" else 1
+    out = np.zeros((rows, e.size))
+    lib().or_prism_magnetic(_p(e), _p(n), _p(u), e.size, _p(prisms), _p(mag),
+                            prisms.shape[0], comp, _p(out), n_threads)
+    out *= 1e9
+    if field == "b":
+        return tuple(o.reshape(shape) for o in out)
+    return out[0].reshape(shape)
+
+
+def prism_magnetic_pair(point, prism, magnetization):
+    """(b_e, b_n, b_u) in T of one prism at one point."""
+    prism = _f64(prism)
+    b = np.zeros(3)
+    lib().or_prism_magnetic_pair(*(float(v) for v in point), _p(prism),
+                                 *(float(v) for v in magnetization), _p(b))
+    return b
 
 
 def num_threads():

@@ -25,6 +25,8 @@
  *   or_eqs_jacobian       src/harmonica/_equivalent_sources/utils.py:54-74
  *   or_eqs_predict        src/harmonica/_equivalent_sources/utils.py:77-97
  *                         (Green's function cartesian.py:634-644)
+ *   or_prism_magnetic     src/harmonica/_forward/prisms/magnetic.py:275-400
+ *                         (per pair: choclo prism.magnetic_field / _e / _n / _u)
  *
  * Pinning (what checks each function against something other than itself)
  * lives in tests/test_oracle_*.py: reference doctest values, the reference's
@@ -166,6 +168,65 @@ double or_prism_pair(int field, double e, double n, double u, const double *p,
     return OR_G * rho * sum;
 }
 
+/* mu_0 / (4 pi) with mu_0 = 4 pi 1e-7 H/m, evaluated in the order
+ * mu_0 / 4 / pi (the magnetic closed forms' constant c_m; choclo's value of
+ * mu_0 is not in this image -- see or_prism_magnetic_pair). */
+#define OR_PI 3.14159265358979311600
+static double or_cm(void)
+{
+    volatile double mu0 = 4.0 * OR_PI * 1e-7;
+    return mu0 / 4.0 / OR_PI;
+}
+
+/* Magnetic field (b_e, b_n, b_u), in T, of one prism p = [W, E, S, N, B, T]
+ * with uniform magnetisation (me, mn, mu) in A/m, at (e, n, u): for each
+ * vertex, with the sign (-1)^(i+j+k) of the gravity sums,
+ *   b_e += me K_ee + mn K_en + mu K_eu
+ *   b_n += me K_en + mn K_nn + mu K_nu
+ *   b_u += me K_eu + mn K_nu + mu K_uu
+ * and b *= c_m at the end: B = (mu_0 / 4 pi) grad (M . grad) int 1/r dV
+ * with the second-derivative kernels of or_kernel in the east-north-up
+ * frame (Poisson's relation; Blakely 1995, ch. 5).  This restates choclo's
+ * prism.magnetic_field / magnetic_e / _n / _u, called per pair at
+ * src/harmonica/_forward/prisms/magnetic.py:319-331,384-397.  On any edge of
+ * the prism (vertices included) all three are NaN. */
+void or_prism_magnetic_pair(double e, double n, double u, const double *p,
+                            double me, double mn, double mu, double *b)
+{
+    if (or_on_easting_edge(e, n, u, p) || or_on_northing_edge(e, n, u, p) ||
+        or_on_upward_edge(e, n, u, p)) {
+        b[0] = b[1] = b[2] = NAN;
+        return;
+    }
+    double be = 0.0, bn = 0.0, bu = 0.0;
+    for (int i = 0; i < 2; ++i) {
+        double x = (i == 0 ? p[1] : p[0]) - e;
+        double x2 = x * x;
+        for (int j = 0; j < 2; ++j) {
+            double y = (j == 0 ? p[3] : p[2]) - n;
+            double y2 = y * y;
+            for (int k = 0; k < 2; ++k) {
+                double z = (k == 0 ? p[5] : p[4]) - u;
+                double r = sqrt(x2 + y2 + z * z);
+                double kee = or_kernel(OR_EE, x, y, z, r);
+                double knn = or_kernel(OR_NN, x, y, z, r);
+                double kuu = or_kernel(OR_UU, x, y, z, r);
+                double ken = or_kernel(OR_EN, x, y, z, r);
+                double keu = or_kernel(OR_EU, x, y, z, r);
+                double knu = or_kernel(OR_NU, x, y, z, r);
+                double s = ((i + j + k) & 1) ? -1.0 : 1.0;
+                be += s * (me * kee + mn * ken + mu * keu);
+                bn += s * (me * ken + mn * knn + mu * knu);
+                bu += s * (me * keu + mn * knu + mu * kuu);
+            }
+        }
+    }
+    double cm = or_cm();
+    b[0] = cm * be;
+    b[1] = cm * bn;
+    b[2] = cm * bu;
+}
+
 static void or_set_threads(int n_threads)
 {
 #ifdef _OPENMP
@@ -189,6 +250,32 @@ void or_prism_gravity(const double *e, const double *n, const double *u,
         for (int64_t j = 0; j < n_prisms; ++j)
             acc += or_prism_pair(field, e[i], n[i], u[i], prisms + 6 * j, rho[j]);
         out[i] = acc;
+    }
+}
+
+/* magnetic.py:275-338 (field "b") and :341-400 (one component): out[c][i]
+ * += component c of or_prism_magnetic_pair over the prisms in index order,
+ * parallel over observations.  comp_mask bit c (0 = b_e, 1 = b_n, 2 = b_u)
+ * selects the rows of out, (popcount(comp_mask), n_obs) row-major.  mag is
+ * (3, m): the east, north and up components.  Results in T (the caller
+ * converts to nT as magnetic.py:195-197,271 do). */
+void or_prism_magnetic(const double *e, const double *n, const double *u,
+                       int64_t n_obs, const double *prisms, const double *mag,
+                       int64_t n_prisms, int comp_mask, double *out, int n_threads)
+{
+    or_set_threads(n_threads);
+    #pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n_obs; ++i) {
+        double acc[3] = {0.0, 0.0, 0.0};
+        for (int64_t j = 0; j < n_prisms; ++j) {
+            double b[3];
+            or_prism_magnetic_pair(e[i], n[i], u[i], prisms + 6 * j, mag[j],
+                                   mag[n_prisms + j], mag[2 * n_prisms + j], b);
+            for (int c = 0; c < 3; ++c) acc[c] += b[c];
+        }
+        int row = 0;
+        for (int c = 0; c < 3; ++c)
+            if (comp_mask & (1 << c)) out[(row++) * n_obs + i] += acc[c];
     }
 }
 

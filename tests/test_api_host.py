@@ -230,3 +230,65 @@ def test_batched_plan_on_host():
     cost = it[:, 3] - it[:, 2]
     assert np.all(np.diff(cost) <= 0)
     assert list(base) == [0, 6, 6] and list(chunks) == [3, 1, 1]
+
+
+# ------------------------------------------------------------ prism_magnetic
+MAG_PRISMS = [[-100, 0, -100, 0, -200, -100], [-100, 0, 0, 100, -200, -100],
+              [0, 100, -100, 0, -200, -100], [0, 100, 0, 100, -200, -100]]
+
+
+def test_magnetic_invalid_field():
+    """test_prism_magnetic.py:29-36."""
+    with pytest.raises(ValueError, match="Invalid field"):
+        hm.prism_magnetic([0, 0, 0], [-100, 100, -100, 100, -200, -100], [1000, 1, 2],
+                          field="Not a valid field")
+    msg = re.escape("Invalid field 'g_z'. Please choose one of 'b,b_e,b_n,b_u'.")
+    with pytest.raises(ValueError, match=msg):
+        hm.prism_magnetic([0, 0, 0], MAG_PRISMS[0], [1.0, 1.0, 1.0], field="g_z")
+
+
+@pytest.mark.parametrize("field", ["b", "b_e", "b_n", "b_u"])
+def test_magnetic_invalid_magnetization(field):
+    """test_prism_magnetic.py:169-225: two vectors for four prisms, five
+    elements per component, four components."""
+    with pytest.raises(ValueError, match="Number of magnetization vectors"):
+        hm.prism_magnetic([0, 0, 0], MAG_PRISMS, ([1.0, -2.0], [1.0, 3.0], [1.0, -5.0]), field)
+    five = ([1.0, 1.0, 1.0, 3.4, 1.3], [-2.0, 3.0, -5.0, 3.4, 1.3], [-2.0, 3.0, -5.0, 3.4, 1.3])
+    with pytest.raises(ValueError, match=re.escape("Number of magnetization vectors (5) "
+                                                   "mismatch the number of prisms (4)")):
+        hm.prism_magnetic([0, 0, 0], MAG_PRISMS, five, field)
+    four = tuple([1.0, 1.0, 1.0, 3.4] for _ in range(4))
+    with pytest.raises(ValueError, match=re.escape("Invalid magnetization vectors with '4' "
+                                                   "elements. Magnetization vectors should have "
+                                                   "only 3 elements.")):
+        hm.prism_magnetic([0, 0, 0], MAG_PRISMS, four, field)
+
+
+@pytest.mark.parametrize("field", ["b", "b_u"])
+@pytest.mark.parametrize("bad", [[100, 0, -100, 0, -200, -100], [-100, 0, 200, 0, -200, -100],
+                                 [-100, 0, -100, 0, -100, -200]])
+def test_magnetic_invalid_prisms(field, bad):
+    """test_prism_magnetic.py:120-158."""
+    with pytest.raises(ValueError, match="boundary can't be greater than the"):
+        hm.prism_magnetic([0, 0, 0], bad, [1.0, 1.0, 1.0], field)
+
+
+def test_magnetic_discard_null_prisms():
+    from harmonica_amd import _magnetic
+
+    prisms = np.array([[0, 10, 0, 10, -10, 0], [5, 5, 0, 10, -10, 0], [0, 10, 0, 10, -10, 0],
+                       [0, 10, 3, 3, -10, 0]], dtype=float)
+    mag = (np.array([1.0, 1.0, 0.0, 2.0]), np.array([0.0, 1.0, 0.0, 2.0]),
+           np.array([0.0, 1.0, 0.0, 2.0]))
+    p, m = _magnetic._discard_null_prisms(prisms, mag)
+    npt.assert_array_equal(p, prisms[:1])
+    npt.assert_array_equal(np.array(m), [[1.0], [0.0], [0.0]])
+
+
+def test_magnetic_empty_coordinates():
+    """No observation points: empty results of the coordinates' shape,
+    without a kernel call."""
+    b = hm.prism_magnetic((np.zeros((0, 3)), np.zeros((0, 3)), np.zeros((0, 3))),
+                          MAG_PRISMS[0], [1.0, 1.0, 1.0], "b")
+    assert len(b) == 3 and all(x.shape == (0, 3) for x in b)
+    assert hm.prism_magnetic(([], [], []), MAG_PRISMS[0], [1.0, 1.0, 1.0], "b_n").shape == (0,)
