@@ -210,6 +210,51 @@ def bench_magnetic(args):
     return out
 
 
+def bench_eqs_gb(args):
+    """Gradient-boosted equivalent sources (EquivalentSourcesGB) on the
+    scattered g_z data of the EQS workload with --gb-data points (default
+    200,000), default windows (about 5000 data each, 50% overlap), damping 10
+    and depth 9 km (the values of the reference's gradient-boosting example,
+    doc/user_guide/equivalent_sources/gradient-boosted-eqs.rst:92-94), FP64
+    window fits; predictions on the 512 x 512 grid.  With --gb-oracle the
+    oracle's CPU loop runs the first 3 windows for a seconds-per-window
+    comparison."""
+    import harmonica_amd as hm
+    from harmonica_amd.workloads import eqs_problem
+
+    damping, depth = 10.0, 9e3
+    w = eqs_problem(n_data=args.gb_data)
+    coords = w["coordinates"]
+    data = hm.prism_gravity(coords, w["prisms"], w["density"], field="g_z") + w["noise"]
+    small = tuple(c[:20000] for c in coords)
+    hm.EquivalentSourcesGB(damping=damping, depth=depth, random_state=0).fit(
+        small, data[:20000])  # untimed warm-up of the code path
+    eqs = hm.EquivalentSourcesGB(damping=damping, depth=depth, random_state=0)
+    _, dt = _sync_time(lambda: eqs.fit(coords, data))
+    pred, dtp = _sync_time(lambda: eqs.predict(w["grid"]))
+    rms = eqs.rmse_per_iteration_
+    out = {"n_data": coords[0].size, "windows": eqs.fit_info_["windows"],
+           "window_size_m": float(eqs.window_size_), "fit_s": dt,
+           "s_per_window": dt / eqs.fit_info_["windows"], "predict_s": dtp,
+           "rmse_first_last": [float(rms[0]), float(rms[-1])], "info": eqs.fit_info_}
+    log(f"eqs gb: {out['n_data']} data, {out['windows']} windows, fit {dt:.2f} s "
+        f"({1e3 * out['s_per_window']:.1f} ms/window), rms {rms[0]:.3g} -> {rms[-1]:.3g}")
+    if args.gb_oracle:
+        import oracle
+
+        src_w, dat_w = eqs._create_windows(coords)
+        t0 = time.perf_counter()
+        _, rms_cpu = oracle.eqs_gb_fit(coords, data, None, eqs.points_, damping, src_w[:3],
+                                       dat_w[:3])
+        cpu_s = (time.perf_counter() - t0) / 3
+        out["oracle_s_per_window"] = cpu_s
+        out["oracle_rmse_after_3"] = float(rms_cpu[-1])
+        out["gpu_rmse_after_3"] = float(rms[3])
+        log(f"eqs gb oracle: {cpu_s:.2f} s/window on {oracle.num_threads()} threads, rms after "
+            f"3 windows {rms_cpu[-1]:.6g} (GPU {rms[3]:.6g})")
+    return out
+
+
 def bench_batched(args):
     """512 independent uneven models (log-uniform 1e3-2e5 prisms, 1e3-5e4
     points), g_z, one persistent graph-captured launch, autotuned."""
@@ -352,6 +397,11 @@ def main():
                     help="skip the equivalent-source fits (FP64 vs mixed, run by default)")
     ap.add_argument("--eqs-all", action="store_true", help="also time the FP32-MFMA mixed fit")
     ap.add_argument("--eqs-data", type=int, default=60000)
+    ap.add_argument("--eqs-gb", action="store_true",
+                    help="also run the gradient-boosted equivalent-source fit")
+    ap.add_argument("--gb-data", type=int, default=200000)
+    ap.add_argument("--gb-oracle", action="store_true",
+                    help="time the oracle's CPU loop on the first 3 windows")
     ap.add_argument("--eqs-oracle", action="store_true", help="check the FP64 fit on a 20k subset")
     ap.add_argument("--inversion", action="store_true", help="also run the autograd inversion")
     ap.add_argument("--eqs-only", choices=("float64", "mixed", "mixed-fp32"),
@@ -461,6 +511,8 @@ def main():
         extra["batched"] = bench_batched(args)
     if not args.no_eqs:
         extra["eqs_fit"] = bench_eqs(args)
+    if args.eqs_gb:
+        extra["eqs_gb"] = bench_eqs_gb(args)
     if args.inversion:
         extra["inversion"] = bench_inversion(args)
 

@@ -7,7 +7,8 @@ Drop-in functions with the reference's Python API (fatiando/harmonica):
 * ``prism_gravity`` (all ten fields, singular-point handling),
 * ``prism_layer`` and its ``.prism_layer`` accessor (``.gravity``),
 * ``point_gravity`` (Cartesian and spherical),
-* ``EquivalentSources`` (jacobian / predict / fit / grid / profile),
+* ``EquivalentSources`` (jacobian / predict / fit / grid / profile) and
+  ``EquivalentSourcesGB`` (gradient-boosted windowed fit),
 * ``prism_magnetic`` (b_e, b_n, b_u or the vector ``b``),
 
 computed by hand-written HIP kernels in libhmx.so (C-ABI in include/hmx.h),
@@ -17,6 +18,7 @@ functions raise.
 """
 
 from ._eqs import EquivalentSources
+from ._eqs_gb import EquivalentSourcesGB
 from ._forward import point_gravity, prism_gravity
 from ._layer import DatasetAccessorPrismLayer, prism_layer
 from ._magnetic import prism_magnetic
@@ -26,6 +28,7 @@ __all__ = [
     "GRAVITATIONAL_CONST",
     "DatasetAccessorPrismLayer",
     "EquivalentSources",
+    "EquivalentSourcesGB",
     "point_gravity",
     "prism_gravity",
     "prism_layer",

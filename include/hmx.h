@@ -287,6 +287,41 @@ int hmx_eqs_fit(const double *easting, const double *northing, const double *upw
                 double *coefs, double *info, void *workspace, size_t workspace_bytes,
                 void *stream);
 
+/* Bytes of workspace needed by hmx_eqs_gb_fit for n_data data and windows
+ * of at most max_window_data data and max_window_sources sources. */
+int hmx_eqs_gb_fit_workspace_bytes(int64_t n_data, int64_t max_window_data,
+                                   int64_t max_window_sources, int precision_mode,
+                                   size_t *bytes);
+
+/* Replaces EquivalentSourcesGB._gradient_boosting
+ * (src/harmonica/_equivalent_sources/gradient_boosted.py:231-283): for each
+ * window in order, the coefficients of its sources by hmx_eqs_fit on the
+ * current residue of its data (jacobian + verde least squares, as
+ * gradient_boosted.py:257-264), then residue -= the prediction of those
+ * sources on every datum (predict_numba_parallel, :266-274) and
+ * coefs[window sources] += the window's coefficients (:280).
+ *   data_index / source_index: device int64 arrays, the windows' data and
+ *     source indices back to back (a window's sources must be distinct);
+ *   data_offset / source_offset: HOST int64 arrays of n_windows + 1
+ *     offsets into them (every window non-empty);
+ *   coefs (n_src), residue (n_data): device outputs, overwritten (coefs
+ *     start from zero, residue from the data);
+ *   rmse: device output of n_windows + 1 values, the RMS of the residue
+ *     before the first window and after each (rmse_per_iteration_, :250,
+ *     :278);
+ *   info: HOST array of 5 doubles: [total CG iterations, last relative
+ *     update, largest shift, total factorisation retries, largest diagonal].
+ * Indices outside [0, n_data) or [0, n_src) give NaN, never an access out of
+ * bounds.  Synchronous. */
+int hmx_eqs_gb_fit(const double *easting, const double *northing, const double *upward,
+                   int64_t n_data, const double *data, const double *weights,
+                   const double *p_easting, const double *p_northing, const double *p_upward,
+                   int64_t n_src, const int64_t *data_index, const int64_t *data_offset,
+                   const int64_t *source_index, const int64_t *source_offset, int64_t n_windows,
+                   double damping, int precision_mode, int max_iter, double rtol, double *coefs,
+                   double *residue, double *rmse, double *info, void *workspace,
+                   size_t workspace_bytes, void *stream);
+
 /* Building blocks of the fit, exposed for testing and reuse (no
  * counterpart in the reference):
  * hmx_mfma_probe: one MFMA of a single wavefront; out[4 l + v] is element

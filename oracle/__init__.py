@@ -11,7 +11,12 @@ they call; see the citations in that file) that reproduce the reference's
 Python-level post-processing: sign flips and unit conversions of
 ``src/harmonica/_forward/prisms/gravity.py:228-235``,
 ``src/harmonica/_forward/prisms/layer.py:425-433`` and
-``src/harmonica/_forward/point.py:253-260``.
+``src/harmonica/_forward/point.py:253-260``, and two Python-level
+restatements: the gradient-boosting window loop of
+``src/harmonica/_equivalent_sources/gradient_boosted.py:231-283`` over the
+oracle's jacobian, least squares and predict, with window membership by
+direct comparison (``rolling_window_windows``), and the null-prism discard
+and nT conversion of ``src/harmonica/_forward/prisms/magnetic.py``.
 """
 
 import ctypes
@@ -246,6 +251,51 @@ def prism_magnetic_pair(point, prism, magnetization):
     lib().or_prism_magnetic_pair(*(float(v) for v in point), _p(prism),
                                  *(float(v) for v in magnetization), _p(b))
     return b
+
+
+def rolling_window_windows(coordinates, window_size, overlap, region):
+    """Window membership by direct comparison (no KD-tree): the points with
+    |easting - c_e| <= size / 2 and |northing - c_n| <= size / 2 for every
+    window centre, centres on a northing-major grid spanning the region
+    shrunk by half a window, (1 - overlap) * size apart rounded to a whole
+    number of steps (bordado.rolling_window as gradient_boosted.py:343-354
+    calls it)."""
+    w, e, s, n = region
+    half = window_size / 2
+    spacing = (1 - overlap) * window_size
+    axes = []
+    for lo, hi in ((w + half, e - half), (s + half, n - half)):
+        axes.append(np.linspace(lo, hi, int(round((hi - lo) / spacing)) + 1))
+    east, north = (np.asarray(c, dtype=np.float64).ravel() for c in coordinates[:2])
+    windows = []
+    for cn in axes[1]:
+        for ce in axes[0]:
+            inside = (np.abs(east - ce) <= half) & (np.abs(north - cn) <= half)
+            windows.append(np.nonzero(inside)[0])
+    return windows
+
+
+def eqs_gb_fit(coordinates, data, weights, points, damping, source_windows, data_windows):
+    """EquivalentSourcesGB._gradient_boosting (gradient_boosted.py:231-283)
+    on the CPU for the given windows: per window the jacobian, verde's least
+    squares on the current residue, the prediction on every datum
+    subtracted, the coefficients accumulated.  Returns (coefs, rmse per
+    iteration)."""
+    coords = tuple(_f64(np.asarray(c).ravel()) for c in coordinates[:3])
+    pts = tuple(_f64(np.asarray(p).ravel()) for p in points[:3])
+    residue = _f64(np.asarray(data).ravel()).copy()
+    coefs = np.zeros(pts[0].size)
+    errors = [np.sqrt(np.mean(residue**2))]
+    for sw, dw in zip(source_windows, data_windows):
+        pchunk = tuple(p[sw] for p in pts)
+        cchunk = tuple(c[dw] for c in coords)
+        wchunk = None if weights is None else np.asarray(weights).ravel()[dw]
+        jac = eqs_jacobian(cchunk, pchunk)
+        cw = eqs_fit_coefs(jac, residue[dw], wchunk, damping)
+        residue -= eqs_predict(coords, pchunk, cw)
+        errors.append(np.sqrt(np.mean(residue**2)))
+        coefs[sw] += cw
+    return coefs, np.array(errors)
 
 
 def num_threads():
