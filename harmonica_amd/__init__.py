@@ -7,8 +7,9 @@ Drop-in functions with the reference's Python API (fatiando/harmonica):
 * ``prism_gravity`` (all ten fields, singular-point handling),
 * ``prism_layer`` and its ``.prism_layer`` accessor (``.gravity``),
 * ``point_gravity`` (Cartesian and spherical),
-* ``EquivalentSources`` (jacobian / predict / fit / grid / profile) and
-  ``EquivalentSourcesGB`` (gradient-boosted windowed fit),
+* ``EquivalentSources`` (jacobian / predict / fit / grid / profile),
+  ``EquivalentSourcesGB`` (gradient-boosted windowed fit) and
+  ``EquivalentSourcesSph`` (spherical coordinates),
 * ``prism_magnetic`` (b_e, b_n, b_u or the vector ``b``),
 
 computed by hand-written HIP kernels in libhmx.so (C-ABI in include/hmx.h),
@@ -19,6 +20,7 @@ functions raise.
 
 from ._eqs import EquivalentSources
 from ._eqs_gb import EquivalentSourcesGB
+from ._eqs_sph import EquivalentSourcesSph
 from ._forward import point_gravity, prism_gravity
 from ._layer import DatasetAccessorPrismLayer, prism_layer
 from ._magnetic import prism_magnetic
@@ -29,6 +31,7 @@ __all__ = [
     "DatasetAccessorPrismLayer",
     "EquivalentSources",
     "EquivalentSourcesGB",
+    "EquivalentSourcesSph",
     "point_gravity",
     "prism_gravity",
     "prism_layer",

@@ -154,6 +154,9 @@ class GridDataset:
 
 # ----------------------------------------------------------- device calls
 def _dev(a, dtype):
+    """Contiguous device tensor of the given dtype (host arrays are copied)."""
+    if isinstance(a, torch.Tensor):
+        return a.to(device=_lib.require_device(), dtype=dtype).contiguous()
     return to_device(np.asarray(a), dtype=dtype)
 
 
@@ -206,9 +209,12 @@ def spatial_order(easting, northing, bits=16):
 
 
 def fit_device(coords, data, weights, points, damping, precision="float64", max_iter=None,
-               rtol=None):
+               rtol=None, order_xy=None):
     """Coefficients of the scaled, damped, weighted least-squares fit on the
-    GPU (all inputs float64).  Returns (coefs numpy array, info dict).
+    GPU (host arrays or device tensors, computed in float64).  Returns
+    (coefs numpy array, info dict).  ``order_xy``: host horizontal
+    coordinates of the data for the spatial order of the split mode (default
+    the first two coordinates).
 
     Every mode refines its solution by conjugate gradients on the FP64
     normal equations, preconditioned by its Cholesky factor: the FP64 mode
@@ -230,14 +236,18 @@ def fit_device(coords, data, weights, points, damping, precision="float64", max_
         # scales separately is a compact region, where the columns of far
         # sources are uniformly small: they keep more significant bits, and
         # the matrix preconditions better.
-        order = spatial_order(coords[0], coords[1])
-        coords = tuple(np.asarray(x)[order] for x in coords)
-        data = np.asarray(data)[order]
-        weights = None if weights is None else np.asarray(weights)[order]
+        xy = coords[:2] if order_xy is None else order_xy
+        order = spatial_order(*(np.asarray(v.cpu() if isinstance(v, torch.Tensor) else v)
+                                for v in xy[:2]))
     c = [_dev(x, torch.float64) for x in coords]
     p = [_dev(x, torch.float64) for x in points]
     d = _dev(data, torch.float64)
     w = None if weights is None else _dev(weights, torch.float64)
+    if mode == _lib.HMX_FIT_BF16_SPLIT:
+        idx = torch.as_tensor(order).to(c[0].device)
+        c = [x[idx].contiguous() for x in c]
+        d = d[idx].contiguous()
+        w = None if w is None else w[idx].contiguous()
     n, m = c[0].numel(), p[0].numel()
     nbytes = ctypes.c_size_t(0)
     check(lib().hmx_eqs_fit_workspace_bytes(n, m, mode, ctypes.byref(nbytes)))

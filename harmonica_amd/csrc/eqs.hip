@@ -113,6 +113,33 @@ int predict_typed(const void *e, const void *n, const void *u, int64_t n_obs, co
 }  // namespace
 }  // namespace hmx
 
+namespace hmx {
+namespace {
+
+// (longitude [deg], latitude [deg], radius [m]) -> geocentric Cartesian
+// (x, y, z) [m].  The spherical equivalent sources' Green's function
+// 1 / l, l^2 = (r - r')^2 + 2 r r' (1 - cos psi) (_forward/utils.py:163-201),
+// is 1 / |X - X'| of these points, evaluated here without the cancellation
+// of 1 - cos psi for nearby points.
+__global__ void geocentric_kernel(const double *__restrict__ lon, const double *__restrict__ lat,
+                                  const double *__restrict__ radius, int64_t n,
+                                  double *__restrict__ x, double *__restrict__ y,
+                                  double *__restrict__ z) {
+    constexpr double kDeg = 3.14159265358979323846 / 180.0;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double sl, cl, sp, cp;
+    sincos(lon[i] * kDeg, &sl, &cl);
+    sincos(lat[i] * kDeg, &sp, &cp);
+    const double r = radius[i];
+    x[i] = r * cp * cl;
+    y[i] = r * cp * sl;
+    z[i] = r * sp;
+}
+
+}  // namespace
+}  // namespace hmx
+
 extern "C" {
 
 int hmx_eqs_jacobian(const void *easting, const void *northing, const void *upward,
@@ -173,6 +200,19 @@ int hmx_eqs_predict(const void *easting, const void *northing, const void *upwar
         return predict_typed<float>(easting, northing, upward, n_obs, p_easting, p_northing,
                                     p_upward, coefs, n_src, out, partial, chunks, s);
     return fail(HMX_ERR_ARG, "invalid dtype %d", dtype);
+}
+
+int hmx_geocentric(const double *longitude, const double *latitude, const double *radius,
+                   int64_t n, double *x, double *y, double *z, void *stream) {
+    using namespace hmx;
+    clear_error();
+    if (n < 0) return fail(HMX_ERR_ARG, "negative size");
+    if (n == 0) return HMX_OK;
+    if (!longitude || !latitude || !radius || !x || !y || !z) return fail(HMX_ERR_ARG, "NULL array");
+    hipLaunchKernelGGL(geocentric_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       (hipStream_t)stream, longitude, latitude, radius, n, x, y, z);
+    HMX_LAUNCHED("geocentric_kernel");
+    return HMX_OK;
 }
 
 }  // extern "C"

@@ -240,6 +240,19 @@ int hmx_batched_prism_gz(const double *easting, const double *northing, const do
                          int blocks_per_cu, double *out, void *workspace, size_t workspace_bytes,
                          void *stream);
 
+/* Spherical equivalent sources (EquivalentSourcesSph,
+ * src/harmonica/_equivalent_sources/spherical.py): their Green's function
+ * greens_func_spherical (spherical.py:412-424) is 1 / l with l the distance
+ * of _forward/utils.py:163-201 between (longitude, latitude, radius)
+ * points, which equals 1 / |X - X'| for the geocentric Cartesian points
+ *   x = r cos(lat) cos(lon), y = r cos(lat) sin(lon), z = r sin(lat).
+ * This writes those points (x, y, z: (n) device outputs [m]; longitude,
+ * latitude in degrees); hmx_eqs_jacobian, hmx_eqs_predict and hmx_eqs_fit
+ * called on them replace jacobian_numba_* / predict_numba_* with the
+ * spherical Green's function and the fit of spherical.py:176-217. */
+int hmx_geocentric(const double *longitude, const double *latitude, const double *radius,
+                   int64_t n, double *x, double *y, double *z, void *stream);
+
 /* Bytes of workspace needed by hmx_eqs_predict on the current device. */
 int hmx_eqs_predict_workspace_bytes(int64_t n_obs, int64_t n_src, size_t *bytes);
 

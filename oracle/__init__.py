@@ -68,6 +68,8 @@ def lib():
         L.or_eqs_predict.argtypes = [dp, dp, dp, i64, dp, dp, dp, dp, i64, dp, ci]
         L.or_prism_magnetic.argtypes = [dp, dp, dp, i64, dp, dp, i64, ci, dp, ci]
         L.or_prism_magnetic_pair.argtypes = [cd, cd, cd, dp, cd, cd, cd, dp]
+        L.or_eqs_sph_jacobian.argtypes = [dp, dp, dp, i64, dp, dp, dp, i64, dp, ci]
+        L.or_eqs_sph_predict.argtypes = [dp, dp, dp, i64, dp, dp, dp, dp, i64, dp, ci]
         L.or_num_threads.restype = ci
         _lib = L
     return _lib
@@ -188,6 +190,27 @@ def eqs_predict(coordinates, points, coefs, n_threads=0):
     out = np.zeros(e.size)
     lib().or_eqs_predict(_p(e), _p(n), _p(u), e.size, _p(pe), _p(pn), _p(pu), _p(c),
                          pe.size, _p(out), n_threads)
+    return out.reshape(shape)
+
+
+def eqs_sph_jacobian(coordinates, points, n_threads=0):
+    """Oracle of ``EquivalentSourcesSph.jacobian`` (spherical.py:250-282)."""
+    c = tuple(_f64(np.atleast_1d(x).ravel()) for x in coordinates[:3])
+    p = tuple(_f64(np.atleast_1d(x).ravel()) for x in points[:3])
+    jac = np.zeros((c[0].size, p[0].size))
+    lib().or_eqs_sph_jacobian(*(_p(x) for x in c), c[0].size, *(_p(x) for x in p), p[0].size,
+                              _p(jac), n_threads)
+    return jac
+
+
+def eqs_sph_predict(coordinates, points, coefs, n_threads=0):
+    """Oracle of ``EquivalentSourcesSph.predict``'s loop (utils.py:77-97)."""
+    shape, c = _coords(coordinates)
+    p = tuple(_f64(np.atleast_1d(x).ravel()) for x in points[:3])
+    cf = _f64(coefs)
+    out = np.zeros(c[0].size)
+    lib().or_eqs_sph_predict(*(_p(x) for x in c), c[0].size, *(_p(x) for x in p), _p(cf),
+                             p[0].size, _p(out), n_threads)
     return out.reshape(shape)
 
 

@@ -25,6 +25,9 @@
  *   or_eqs_jacobian       src/harmonica/_equivalent_sources/utils.py:54-74
  *   or_eqs_predict        src/harmonica/_equivalent_sources/utils.py:77-97
  *                         (Green's function cartesian.py:634-644)
+ *   or_eqs_sph_jacobian   src/harmonica/_equivalent_sources/utils.py:54-74
+ *   or_eqs_sph_predict    src/harmonica/_equivalent_sources/utils.py:77-97
+ *                         (Green's function spherical.py:412-424)
  *   or_prism_magnetic     src/harmonica/_forward/prisms/magnetic.py:275-400
  *                         (per pair: choclo prism.magnetic_field / _e / _n / _u)
  *
@@ -352,6 +355,53 @@ double or_point_pair(int field, double e, double n, double u, double ep,
     case OR_NU: return OR_G * m * 3 * dn * du / d5;
     }
     return 0.0;
+}
+
+/* The distance of _forward/utils.py:121-201 between two (longitude,
+ * latitude, radius) points, angles in degrees: the Green's function of the
+ * spherical equivalent sources is its inverse (spherical.py:412-424). */
+static double or_distance_spherical(double lon, double lat, double r, double plon,
+                                    double plat, double pr)
+{
+    const double deg = 3.14159265358979323846 / 180.0;
+    double la = lat * deg, pla = plat * deg;
+    double cosphi_p = cos(pla), sinphi_p = sin(pla);
+    double cosphi = cos(la), sinphi = sin(la);
+    double coslambda = cos(plon * deg - lon * deg);
+    double cospsi = sinphi_p * sinphi + cosphi_p * cosphi * coslambda;
+    double dr = r - pr;
+    return sqrt(dr * dr + 2.0 * r * pr * (1.0 - cospsi));
+}
+
+/* utils.py:54-74 with greens_func_spherical (spherical.py:412-424):
+ * jac[i * m + j] = 1 / l(point_i, source_j). */
+void or_eqs_sph_jacobian(const double *lon, const double *lat, const double *rad,
+                         int64_t n, const double *plon, const double *plat,
+                         const double *prad, int64_t m, double *jac, int n_threads)
+{
+    or_set_threads(n_threads);
+    #pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i)
+        for (int64_t j = 0; j < m; ++j)
+            jac[i * m + j] = 1.0 / or_distance_spherical(lon[i], lat[i], rad[i], plon[j],
+                                                         plat[j], prad[j]);
+}
+
+/* utils.py:77-97 with greens_func_spherical: out[i] += sum_j c_j / l. */
+void or_eqs_sph_predict(const double *lon, const double *lat, const double *rad,
+                        int64_t n, const double *plon, const double *plat,
+                        const double *prad, const double *coefs, int64_t m,
+                        double *out, int n_threads)
+{
+    or_set_threads(n_threads);
+    #pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) {
+        double acc = out[i];
+        for (int64_t j = 0; j < m; ++j)
+            acc += coefs[j] / or_distance_spherical(lon[i], lat[i], rad[i], plon[j], plat[j],
+                                                    prad[j]);
+        out[i] = acc;
+    }
 }
 
 /* point.py:357-398 */
