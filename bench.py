@@ -255,6 +255,29 @@ def bench_eqs_gb(args):
     return out
 
 
+def bench_tesseroid(args):
+    """tesseroid_gravity g_z of a global 1 x 1 degree tesseroid layer
+    (64,800 tesseroids, tops from a fractal surface in [-4, 4] km about the
+    mean Earth radius, bottoms 30 km below the mean radius) on a 1 x 1
+    degree grid of 65,160 points 10 km above the mean radius (2D adaptive
+    discretisation, distance-size ratio 2.5)."""
+    import harmonica_amd as hm
+    from harmonica_amd.workloads import global_tesseroid_layer
+
+    w = global_tesseroid_layer()
+    tess, dens, coords = w["tesseroids"], w["density"], w["coordinates"]
+    pairs = coords[0].size * tess.shape[0]
+    hm.tesseroid_gravity(tuple(c[:2, :] for c in coords), tess, dens, field="g_z",
+                         disable_checks=True)  # untimed warm-up
+    _, dt = _sync_time(lambda: hm.tesseroid_gravity(coords, tess, dens, field="g_z",
+                                                    disable_checks=True))
+    out = {"n_points": coords[0].size, "n_tesseroids": tess.shape[0], "s": dt,
+           "pair_evals_per_s": pairs / dt}
+    log(f"tesseroid g_z: {coords[0].size} points x {tess.shape[0]} tesseroids, {dt:.3f} s, "
+        f"{pairs / dt:.4e} pair-evals/s")
+    return out
+
+
 def bench_batched(args):
     """512 independent uneven models (log-uniform 1e3-2e5 prisms, 1e3-5e4
     points), g_z, one persistent graph-captured launch, autotuned."""
@@ -391,6 +414,8 @@ def main():
     ap.add_argument("--point", action="store_true", help="also time point_gravity (1M masses)")
     ap.add_argument("--magnetic", action="store_true",
                     help="also time prism_magnetic (field b) on the 1M terrain prisms")
+    ap.add_argument("--tesseroid", action="store_true",
+                    help="also time tesseroid_gravity on a global 1-degree layer")
     ap.add_argument("--batched", action="store_true", help="also run the 512-model batch")
     ap.add_argument("--batched-models", type=int, default=512)
     ap.add_argument("--no-eqs", action="store_true",
@@ -507,6 +532,8 @@ def main():
         extra["point_masses"] = bench_point(args)
     if args.magnetic:
         extra["magnetic"] = bench_magnetic(args)
+    if args.tesseroid:
+        extra["tesseroid"] = bench_tesseroid(args)
     if args.batched:
         extra["batched"] = bench_batched(args)
     if not args.no_eqs:

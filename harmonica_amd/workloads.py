@@ -16,6 +16,10 @@ reference's tests.
   induced by a 50,000 nT field (inclination -30, declination 12 degrees)
   with fractal susceptibility in [0.001, 0.05] SI, plus N(0, 0.2 A/m)
   remanence per component.
+* global_tesseroid_layer  64,800 tesseroids of 1 x 1 degree over the globe
+  (tops from a fractal surface in [-4, 4] km about the mean Earth radius,
+  bottoms 30 km below it, densities 2670 +- 200 kg/m^3) and a 1-degree grid
+  of 65,160 points 10 km above the mean radius.
 * eqs_problem          60,000 scattered g_z data from 3 prisms + 1 mGal noise.
 * inversion_layer      200 x 200 basement layer and a 128 x 128 grid at 500 m.
 """
@@ -93,6 +97,24 @@ def terrain_magnetization(n_prisms_side=1000, seed=5):
     h0 = 50000e-9 / (4e-7 * np.pi)  # |B0| / mu_0 in A/m
     unit = (np.cos(inc) * np.sin(dec), np.cos(inc) * np.cos(dec), -np.sin(inc))
     return tuple(chi * h0 * d + rng.normal(0.0, 0.2, chi.size) for d in unit)
+
+
+def global_tesseroid_layer(seed=6, spacing=1.0):
+    rng = np.random.default_rng(seed)
+    mean_radius = 6371008.771415059  # WGS84 mean radius
+    lon = np.arange(-180.0, 180.0, spacing)
+    lat = np.arange(-90.0, 90.0, spacing)
+    n_lon, n_lat = lon.size, lat.size
+    tops = fractal_surface(max(n_lon, n_lat), rng, -4e3, 4e3)[:n_lat, :n_lon]
+    W, S = np.meshgrid(lon, lat)
+    tess = np.column_stack([W.ravel(), (W + spacing).ravel(), S.ravel(), (S + spacing).ravel(),
+                            np.full(W.size, mean_radius - 3e4), (mean_radius + tops).ravel()])
+    density = rng.uniform(2470.0, 2870.0, W.size)
+    glon = np.linspace(-180.0, 179.0, 360)
+    glat = np.linspace(-90.0, 90.0, 181)
+    GL, GT = np.meshgrid(glon, glat)
+    return {"tesseroids": tess, "density": density,
+            "coordinates": (GL, GT, np.full(GL.shape, mean_radius + 1e4))}
 
 
 def layer_to_prisms(easting, northing, surface, reference):

@@ -154,6 +154,30 @@ int hmx_prism_magnetic(const double *easting, const double *northing, const doub
                        int64_t n_prisms, uint32_t component_mask, double *out, void *workspace,
                        size_t workspace_bytes, void *stream);
 
+/* Bytes of workspace needed by hmx_tesseroid_gravity. */
+int hmx_tesseroid_workspace_bytes(int64_t n_obs, int64_t n_tesseroids, size_t *bytes);
+
+/* Replaces jit_tesseroid_gravity and jit_tesseroid_gravity_variable_density
+ * (src/harmonica/_forward/tesseroid_gravity.py:236-452) with the adaptive
+ * discretisation and quadrature of _tesseroid_utils.py, plus the sign flip
+ * and mGal conversion of g_z (tesseroid_gravity.py:207-211).
+ *   longitude, latitude [deg], radius [m]: (n_obs) observation points;
+ *   tesseroids: (n_tesseroids, 6) row-major [west, east, south, north,
+ *     bottom, top] with west <= east (the Python layer applies the
+ *     reference's checks and longitude continuity);
+ *   density_nodes: (n_tesseroids, 2) density at the lower and upper radial
+ *     Gauss-Legendre node (both the tesseroid's density for a constant one);
+ *     with radial_discretization the density must be constant;
+ *   field: HMX_POTENTIAL or HMX_G_Z; out: (n_obs), overwritten;
+ *   overflow: device int, set to 1 (stack of 100 exceeded) or 2 (more than
+ *     100000 small tesseroids for a pair) as the reference's OverflowError;
+ *     the caller clears it. */
+int hmx_tesseroid_gravity(const double *longitude, const double *latitude, const double *radius,
+                          int64_t n_obs, const double *tesseroids, const double *density_nodes,
+                          int64_t n_tesseroids, uint32_t field, int radial_discretization,
+                          double *out, int *overflow, void *workspace, size_t workspace_bytes,
+                          void *stream);
+
 /* Replaces point_mass_cartesian / point_mass_spherical
  * (src/harmonica/_forward/point.py:357-454) with the choclo.point kernels
  * (point.py:281-316) or the spherical kernels (point.py:324-354), plus the

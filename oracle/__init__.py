@@ -70,6 +70,10 @@ def lib():
         L.or_prism_magnetic_pair.argtypes = [cd, cd, cd, dp, cd, cd, cd, dp]
         L.or_eqs_sph_jacobian.argtypes = [dp, dp, dp, i64, dp, dp, dp, i64, dp, ci]
         L.or_eqs_sph_predict.argtypes = [dp, dp, dp, i64, dp, dp, dp, dp, i64, dp, ci]
+        L.or_tesseroid_gravity.argtypes = [dp, dp, dp, i64, dp, dp, i64, ci, cd, ci, dp, ci]
+        L.or_tesseroid_gravity.restype = ci
+        L.or_tesseroid_split_list.argtypes = [cd, cd, cd, dp, cd, ci, ci, dp, ctypes.c_long]
+        L.or_tesseroid_split_list.restype = ctypes.c_long
         L.or_num_threads.restype = ci
         _lib = L
     return _lib
@@ -319,6 +323,61 @@ def eqs_gb_fit(coordinates, data, weights, points, damping, source_windows, data
         errors.append(np.sqrt(np.mean(residue**2)))
         coefs[sw] += cw
     return coefs, np.array(errors)
+
+
+#: Distance-size ratios of the adaptive discretisation (tesseroid_gravity.py:32).
+TESSEROID_RATIOS = {"potential": 1.0, "g_z": 2.5}
+#: Mean radius of the WGS84 ellipsoid (boule.WGS84.mean_radius), used by the
+#: reference's tesseroid tests.
+WGS84_MEAN_RADIUS = 6371008.771415059
+
+
+def tesseroid_gravity(coordinates, tesseroids, density, field, radial=False, n_threads=0):
+    """Oracle of ``harmonica.tesseroid_gravity`` (tesseroid_gravity.py:35-213)
+    for constant densities, without the input checks: west > east boundaries
+    moved to [-180, 180) (_tesseroid_utils.py:_longitude_continuity), null
+    tesseroids discarded, g_z flipped and in mGal.  ``density`` may also be
+    an (m, 2) array of densities at the lower and upper radial quadrature
+    nodes of each tesseroid (variable density without radial
+    discretisation).  Raises OverflowError as the reference does."""
+    shape, (lon, lat, rad) = _coords(coordinates)
+    t = _f64(np.atleast_2d(tesseroids)).copy()
+    swap = t[:, 0] > t[:, 1]
+    t[swap, 1] = ((t[swap, 1] + 180) % 360) - 180
+    t[swap, 0] = ((t[swap, 0] + 180) % 360) - 180
+    rho = np.asarray(density, dtype=np.float64)
+    rho = np.repeat(np.atleast_1d(rho).ravel()[:, None], 2, axis=1) if rho.ndim < 2 else rho
+    null = ((t[:, 0] == t[:, 1]) | (t[:, 2] == t[:, 3]) | (t[:, 4] == t[:, 5])
+            | np.all(rho == 0, axis=1))
+    t, rho = _f64(t[~null]), _f64(rho[~null])
+    out = np.zeros(lon.size)
+    st = lib().or_tesseroid_gravity(_p(lon), _p(lat), _p(rad), lon.size, _p(t), _p(rho),
+                                    t.shape[0], OR_FIELD_TESS[field], TESSEROID_RATIOS[field],
+                                    int(bool(radial)), _p(out), n_threads)
+    if st == 1:
+        raise OverflowError("Stack Overflow. Try to increase the stack size.")
+    if st == 2:
+        raise OverflowError("Exceeded maximum discretizations. Please increase the "
+                            "MAX_DISCRETIZATIONS.")
+    if field == "g_z":
+        out *= -1e5
+    return out.reshape(shape)
+
+
+OR_FIELD_TESS = {"potential": 0, "g_z": 3}
+
+
+def tesseroid_split_list(point, tesseroid, ratio, radial=False, stack_size=100, max_out=100000):
+    """Small tesseroids of one pair in production order
+    (_tesseroid_utils.py:_adaptive_discretization); raises OverflowError."""
+    out = np.zeros((max_out, 6))
+    t = _f64(tesseroid)
+    n = lib().or_tesseroid_split_list(float(point[0]), float(point[1]), float(point[2]), _p(t),
+                                      float(ratio), int(bool(radial)), int(stack_size),
+                                      _p(out), int(max_out))
+    if n < 0:
+        raise OverflowError("discretisation overflow")
+    return out[:n]
 
 
 def num_threads():

@@ -28,6 +28,8 @@
  *   or_eqs_sph_jacobian   src/harmonica/_equivalent_sources/utils.py:54-74
  *   or_eqs_sph_predict    src/harmonica/_equivalent_sources/utils.py:77-97
  *                         (Green's function spherical.py:412-424)
+ *   or_tesseroid_gravity  src/harmonica/_forward/tesseroid_gravity.py:236-336,
+ *                         _tesseroid_utils.py (discretisation, quadrature)
  *   or_prism_magnetic     src/harmonica/_forward/prisms/magnetic.py:275-400
  *                         (per pair: choclo prism.magnetic_field / _e / _n / _u)
  *
@@ -402,6 +404,204 @@ void or_eqs_sph_predict(const double *lon, const double *lat, const double *rad,
                                                     prad[j]);
         out[i] = acc;
     }
+}
+
+/* ---------------------------------------------------------------- tesseroids
+ * src/harmonica/_forward/tesseroid_gravity.py:236-336 and
+ * _tesseroid_utils.py: per point and tesseroid, adaptive discretisation with
+ * a stack (STACK_SIZE 100, at most MAX_DISCRETIZATIONS 100000 small
+ * tesseroids), each small tesseroid integrated by 2 x 2 x 2 Gauss-Legendre
+ * quadrature as point masses with the spherical point-mass kernels. */
+#define OR_TESS_STACK 100
+#define OR_TESS_MAX 100000
+#define OR_RAD(x) ((x) * (3.14159265358979323846 / 180.0))
+
+/* _tesseroid_utils.py:_tesseroid_dimensions */
+static void or_tess_dimensions(const double *t, double *l_lon, double *l_lat, double *l_rad)
+{
+    double w = OR_RAD(t[0]), e = OR_RAD(t[1]), s = OR_RAD(t[2]), n = OR_RAD(t[3]);
+    double top = t[5], bottom = t[4];
+    double lat_c = (n + s) / 2;
+    *l_lat = top * acos(sin(n) * sin(s) + cos(n) * cos(s));
+    double sc = sin(lat_c), cc = cos(lat_c);
+    *l_lon = top * acos(sc * sc + cc * cc * cos(e - w));
+    *l_rad = top - bottom;
+}
+
+/* _tesseroid_utils.py:_distance_tesseroid_point (distance_spherical to the
+ * tesseroid's centre) */
+static double or_tess_distance(double lon, double lat, double r, const double *t)
+{
+    return or_distance_spherical(lon, lat, r, (t[0] + t[1]) / 2, (t[2] + t[3]) / 2,
+                                 (t[4] + t[5]) / 2);
+}
+
+/* _tesseroid_utils.py:gauss_legendre_quadrature, degree 2 in each direction
+ * (nodes -+1/sqrt(3), weights 1), with the density at the two radial nodes
+ * (rho[0] for the lower node, rho[1] for the upper: the variable-density
+ * quadrature of _tesseroid_variable_density.py evaluates the density there). */
+static double or_tess_glq(double lon_rad, double cosphi, double sinphi, double r,
+                          const double *t, const double *rho, int field)
+{
+    /* numpy.polynomial.legendre.leggauss(2): nodes -+0x1.279a74590331cp-1, weights 1 */
+    static const double node[2] = {-0x1.279a74590331cp-1, 0x1.279a74590331cp-1};
+    double w = t[0], e = t[1], s = t[2], n = t[3], bottom = t[4], top = t[5];
+    double a_factor = 1.0 / 8.0 * OR_RAD(e - w) * OR_RAD(n - s) * (top - bottom);
+    double result = 0.0;
+    for (int j = 0; j < 2; ++j) {
+        double lat_p = OR_RAD(0.5 * (n - s) * node[j] + 0.5 * (n + s));
+        double cosphi_p = cos(lat_p), sinphi_p = sin(lat_p);
+        for (int k = 0; k < 2; ++k) {
+            double radius_p = 0.5 * (top - bottom) * node[k] + 0.5 * (top + bottom);
+            double kappa = radius_p * radius_p * cosphi_p;
+            for (int i = 0; i < 2; ++i) {
+                double lon_p = OR_RAD(0.5 * (e - w) * node[i] + 0.5 * (e + w));
+                double mass = rho[k] * a_factor * kappa * 1.0 * 1.0 * 1.0;
+                double coslambda = cos(lon_p - lon_rad);
+                double cospsi = sinphi_p * sinphi + cosphi_p * cosphi * coslambda;
+                double dr = r - radius_p;
+                double dist = sqrt(dr * dr + 2.0 * r * radius_p * (1.0 - cospsi));
+                double k_val;
+                if (field == OR_POT) {
+                    k_val = 1.0 / dist * OR_G;
+                } else {
+                    double dz = r - radius_p * cospsi;
+                    k_val = -OR_G * dz / (dist * dist * dist);
+                }
+                result += mass * k_val;
+            }
+        }
+    }
+    return result;
+}
+
+/* The contribution of tesseroid t at the point (adaptive discretisation of
+ * _tesseroid_utils.py:_adaptive_discretization then quadrature in the order
+ * the small tesseroids are produced).  *overflow gets 1 (stack) or 2 (too
+ * many small tesseroids), as the reference's OverflowError. */
+static double or_tess_pair(double lon, double lat, double r, double lon_rad, double cosphi,
+                           double sinphi, const double *t0, const double *rho, int field,
+                           double ratio, int radial, int *overflow)
+{
+    double stack[OR_TESS_STACK][6];
+    int top = 0;
+    long n_small = 0;
+    double result = 0.0;
+    for (int c = 0; c < 6; ++c) stack[0][c] = t0[c];
+    while (top >= 0) {
+        double t[6];
+        for (int c = 0; c < 6; ++c) t[c] = stack[top][c];
+        --top;
+        double l_lon, l_lat, l_rad;
+        or_tess_dimensions(t, &l_lon, &l_lat, &l_rad);
+        double d = or_tess_distance(lon, lat, r, t);
+        int n_lon = 1, n_lat = 1, n_rad = 1;
+        if (d / l_lon < ratio) n_lon = 2;
+        if (d / l_lat < ratio) n_lat = 2;
+        if (d / l_rad < ratio && radial) n_rad = 2;
+        if (n_lon * n_lat * n_rad > 1) {
+            if ((top + 1) + n_lon * n_lat * n_rad > OR_TESS_STACK) {
+                *overflow = 1;
+                return NAN;
+            }
+            /* _split_tesseroid */
+            double d_lon = (t[1] - t[0]) / n_lon, d_lat = (t[3] - t[2]) / n_lat;
+            double d_rad = (t[5] - t[4]) / n_rad;
+            for (int i = 0; i < n_lon; ++i)
+                for (int j = 0; j < n_lat; ++j)
+                    for (int k = 0; k < n_rad; ++k) {
+                        ++top;
+                        stack[top][0] = t[0] + d_lon * i;
+                        stack[top][1] = t[0] + d_lon * (i + 1);
+                        stack[top][2] = t[2] + d_lat * j;
+                        stack[top][3] = t[2] + d_lat * (j + 1);
+                        stack[top][4] = t[4] + d_rad * k;
+                        stack[top][5] = t[4] + d_rad * (k + 1);
+                    }
+        } else {
+            if (n_small + 1 > OR_TESS_MAX) {
+                *overflow = 2;
+                return NAN;
+            }
+            ++n_small;
+            result += or_tess_glq(lon_rad, cosphi, sinphi, r, t, rho, field);
+        }
+    }
+    return result;
+}
+
+/* tesseroid_gravity.py:236-336: out[i] += sum_j (tesseroid j at point i),
+ * field OR_POT or OR_U (SI, before the reference's sign flip and mGal).
+ * tess: (m, 6); rho: (m, 2) densities at the lower and upper radial nodes.
+ * Returns 0, or 1 / 2 on a stack / discretisation overflow. */
+int or_tesseroid_gravity(const double *lon, const double *lat, const double *rad,
+                         int64_t n_obs, const double *tess, const double *rho, int64_t m,
+                         int field, double ratio, int radial, double *out, int n_threads)
+{
+    int overflow = 0;
+    or_set_threads(n_threads);
+    #pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t i = 0; i < n_obs; ++i) {
+        double lon_rad = OR_RAD(lon[i]);
+        double cosphi = cos(OR_RAD(lat[i])), sinphi = sin(OR_RAD(lat[i]));
+        double acc = out[i];
+        int ov = 0;
+        for (int64_t j = 0; j < m && !ov; ++j)
+            acc += or_tess_pair(lon[i], lat[i], rad[i], lon_rad, cosphi, sinphi, tess + 6 * j,
+                                rho + 2 * j, field, ratio, radial, &ov);
+        out[i] = acc;
+        if (ov) {
+            #pragma omp atomic write
+            overflow = ov;
+        }
+    }
+    return overflow;
+}
+
+/* The small tesseroids of one (point, tesseroid) pair in production order
+ * (for tests of the discretisation); returns their count, -1 / -2 on
+ * overflow, at most max_out rows written to out (n, 6). */
+long or_tesseroid_split_list(double lon, double lat, double r, const double *t0, double ratio,
+                             int radial, int stack_size, double *out, long max_out)
+{
+    double stack[OR_TESS_STACK][6];
+    if (stack_size > OR_TESS_STACK) stack_size = OR_TESS_STACK;
+    int top = 0;
+    long n_small = 0;
+    for (int c = 0; c < 6; ++c) stack[0][c] = t0[c];
+    while (top >= 0) {
+        double t[6];
+        for (int c = 0; c < 6; ++c) t[c] = stack[top][c];
+        --top;
+        double l_lon, l_lat, l_rad;
+        or_tess_dimensions(t, &l_lon, &l_lat, &l_rad);
+        double d = or_tess_distance(lon, lat, r, t);
+        int n_lon = 1, n_lat = 1, n_rad = 1;
+        if (d / l_lon < ratio) n_lon = 2;
+        if (d / l_lat < ratio) n_lat = 2;
+        if (d / l_rad < ratio && radial) n_rad = 2;
+        if (n_lon * n_lat * n_rad > 1) {
+            if ((top + 1) + n_lon * n_lat * n_rad > stack_size) return -1;
+            double d_lon = (t[1] - t[0]) / n_lon, d_lat = (t[3] - t[2]) / n_lat;
+            double d_rad = (t[5] - t[4]) / n_rad;
+            for (int i = 0; i < n_lon; ++i)
+                for (int j = 0; j < n_lat; ++j)
+                    for (int k = 0; k < n_rad; ++k) {
+                        ++top;
+                        stack[top][0] = t[0] + d_lon * i;
+                        stack[top][1] = t[0] + d_lon * (i + 1);
+                        stack[top][2] = t[2] + d_lat * j;
+                        stack[top][3] = t[2] + d_lat * (j + 1);
+                        stack[top][4] = t[4] + d_rad * k;
+                        stack[top][5] = t[4] + d_rad * (k + 1);
+                    }
+        } else {
+            if (n_small + 1 > max_out) return -2;
+            for (int c = 0; c < 6; ++c) out[6 * n_small + c] = t[c];
+            ++n_small;
+        }
+    }
+    return n_small;
 }
 
 /* point.py:357-398 */
