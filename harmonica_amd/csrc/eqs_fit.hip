@@ -782,6 +782,7 @@ __global__ void zero_int_kernel(int *p) {
 constexpr int kSlices = 4;
 constexpr int kFoldChunks = 32;  // 32 chunks of 32 rows: order-0 sums stay exact
 constexpr int kFoldRows = 32 * kFoldChunks;
+static_assert(kFoldChunks % 2 == 0, "the gram kernel alternates its LDS buffers in pairs of chunks");
 
 // f[G][j] = 2^(E + 1) with max_{k in group G} sqrt(w_k) J_kj / s_j < 2^E
 // (1 if all zero), for the groups G of kFoldRows rows.
@@ -870,7 +871,8 @@ __global__ __launch_bounds__(256) void slice_kernel(
 // tile (2 x 4 waves of 64 x 32, 4 x 2 MFMA 16x16x32 bf16 tiles each) and
 // one FP32 accumulator set per order, folded into FP64 every kFoldChunks
 // chunks of 32 rows.  Slices 0..O1 of both panels go through two LDS
-// buffers with the loads of the next chunk in flight during the MFMAs.
+// buffers, loaded straight from global memory into LDS, with the loads of
+// the next chunk in flight during the MFMAs.
 template <int O0, int O1>
 __global__ __launch_bounds__(512) void gram_bf16_kernel(const bf16x8_t *__restrict__ S,
                                                         int64_t n_groups, int64_t m,
@@ -880,7 +882,10 @@ __global__ __launch_bounds__(512) void gram_bf16_kernel(const bf16x8_t *__restri
                                                         int64_t tiles_n) {
     constexpr int kNO = O1 - O0 + 1;  // orders accumulated here
     constexpr int kNS = O1 + 1;       // slices needed on each side
-    __shared__ bf16x8_t lds[2][2][kNS][4][kST];
+    // Two staging buffers as separate arrays, so that the compiler can tell
+    // the buffer being filled from the one being read.
+    __shared__ bf16x8_t lds0[2][kNS][4][kST];
+    __shared__ bf16x8_t lds1[2][kNS][4][kST];
     int64_t rb, cb;
     if (!upper_tile_xcd(blockIdx.x, tiles_n, rb, cb)) return;
     const int64_t r0 = rb * kST, c0 = cb * kST;
@@ -893,31 +898,54 @@ __global__ __launch_bounds__(512) void gram_bf16_kernel(const bf16x8_t *__restri
     for (int ti = 0; ti < 4; ++ti)
 #pragma unroll
         for (int tj = 0; tj < 2; ++tj) dacc[ti][tj] = double4_t{0, 0, 0, 0};
-    // staging: per side and slice, 4 k-groups x kST columns = 1 unit/thread
+    // Staging: per side and slice, 4 k-groups x kST columns, one 16-byte unit
+    // per thread, loaded straight into LDS (global_load_lds_dwordx4: lane l
+    // of a wave writes 16 bytes at the wave's base + 16 l, so each wave fills
+    // one contiguous run of 64 columns).  Columns past m load column m - 1;
+    // the products they enter belong to output columns past m, never stored.
     const int kg = threadIdx.x >> 7, col = threadIdx.x & (kST - 1);
-    bf16x8_t pre[2][kNS];
-    auto fetch = [&](int ch) {
+    const int col0 = col & ~63;
+    auto issue = [&](int ch, bf16x8_t (&L)[2][kNS][4][kST]) {
         const int64_t g = 4 * (int64_t)ch + kg;
 #pragma unroll
         for (int side = 0; side < 2; ++side) {
-            const int64_t gc = (side ? c0 : r0) + col;
+            int64_t gc = (side ? c0 : r0) + col;
+            gc = gc < m ? gc : m - 1;
 #pragma unroll
             for (int t = 0; t < kNS; ++t)
-                pre[side][t] = gc < m ? S[((int64_t)t * n_groups + g) * m + gc] : zero8;
+                __builtin_amdgcn_global_load_lds(
+                    (const void *)(S + ((int64_t)t * n_groups + g) * m + gc),
+                    (__attribute__((address_space(3))) void *)&L[side][t][kg][col0], 16, 0, 0);
         }
     };
-    auto store = [&](int buf) {
+    // MFMAs of one chunk from the staged buffer L.
+    auto compute = [&](const bf16x8_t (&L)[2][kNS][4][kST]) {
 #pragma unroll
-        for (int side = 0; side < 2; ++side)
+        for (int o = 0; o < kNO; ++o)
 #pragma unroll
-            for (int t = 0; t < kNS; ++t) lds[buf][side][t][kg][col] = pre[side][t];
+            for (int sa = 0; sa <= O0 + o; ++sa) {
+                const int sb = O0 + o - sa;
+                bf16x8_t a[4], bb[2];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) a[t] = L[0][sa][lane >> 4][64 * wr + 16 * t + (lane & 15)];
+#pragma unroll
+                for (int t = 0; t < 2; ++t) bb[t] = L[1][sb][lane >> 4][32 * wc + 16 * t + (lane & 15)];
+#pragma unroll
+                for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+                    for (int tj = 0; tj < 2; ++tj)
+                        acc[o][ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                            a[ti], bb[tj], acc[o][ti][tj], 0, 0, 0);
+            }
+    };
+    // The chunk's loads complete (this wave's), then every wave's.
+    auto arrive = [&]() {
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
     };
     const int chunks = __builtin_amdgcn_readfirstlane((int)(n_groups / 4));
-    if (chunks > 0) {
-        fetch(0);
-        store(0);
-    }
-    __syncthreads();
+    if (chunks > 0) issue(0, lds0);
+    arrive();
     for (int g0 = 0; g0 < chunks; g0 += kFoldChunks) {
         const int g1 = g0 + kFoldChunks < chunks ? g0 + kFoldChunks : chunks;
         // accumulators born in the matrix unit, so they stay in its registers
@@ -929,31 +957,21 @@ __global__ __launch_bounds__(512) void gram_bf16_kernel(const bf16x8_t *__restri
                 for (int tj = 0; tj < 2; ++tj)
                     acc[o][ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
                         zero8, zero8, float4_t{0, 0, 0, 0}, 0, 0, 0);
-        for (int ch = g0; ch < g1; ++ch) {
-            const int cur = ch & 1;
-            const bool more = ch + 1 < chunks;
-            if (more) fetch(ch + 1);
-#pragma unroll
-            for (int o = 0; o < kNO; ++o)
-#pragma unroll
-                for (int sa = 0; sa <= O0 + o; ++sa) {
-                    const int sb = O0 + o - sa;
-                    bf16x8_t a[4], bb[2];
-#pragma unroll
-                    for (int t = 0; t < 4; ++t)
-                        a[t] = lds[cur][0][sa][lane >> 4][64 * wr + 16 * t + (lane & 15)];
-#pragma unroll
-                    for (int t = 0; t < 2; ++t)
-                        bb[t] = lds[cur][1][sb][lane >> 4][32 * wc + 16 * t + (lane & 15)];
-#pragma unroll
-                    for (int ti = 0; ti < 4; ++ti)
-#pragma unroll
-                        for (int tj = 0; tj < 2; ++tj)
-                            acc[o][ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                                a[ti], bb[tj], acc[o][ti][tj], 0, 0, 0);
-                }
-            if (more) store(cur ^ 1);
-            __syncthreads();
+        // kFoldChunks is even, so every fold starts in lds0; pairs of
+        // chunks alternate lds0 and lds1 with the buffer known statically.
+        int ch = g0;
+        for (; ch + 1 < g1; ch += 2) {
+            issue(ch + 1, lds1);
+            compute(lds0);
+            arrive();
+            if (ch + 2 < chunks) issue(ch + 2, lds0);
+            compute(lds1);
+            arrive();
+        }
+        if (ch < g1) {  // odd tail (last fold only): chunk ch in lds0
+            if (ch + 1 < chunks) issue(ch + 1, lds1);
+            compute(lds0);
+            arrive();
         }
         // fold: the group's exact order sums, scaled by f_Gr f_Gc (powers of 2)
         const double *fg = f + (int64_t)(g0 / kFoldChunks) * m;
