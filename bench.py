@@ -157,6 +157,7 @@ def bench_point(args):
     fields fused (hmx_point_gravity, inputs in HBM)."""
     from harmonica_amd._forward import point_fields_device, to_device
     from harmonica_amd._lib import FIELDS
+    from harmonica_amd import workloads
     from harmonica_amd.workloads import layer_to_prisms, terrain_layer_large
 
     w = terrain_layer_large(n_prisms_side=args.prism_side, n_obs_side=args.obs_side)
@@ -401,6 +402,77 @@ def bench_inversion(args):
             "iterations": len(hist) - 1}
 
 
+def bench_cpu_configs(args):
+    """The oracle's CPU loop beside the GPU on the other workloads: the
+    100 x 100 terrain layer with 64 x 64 points (all 4.1e7 pairs, layer
+    path, g_z), a 16-model subset of the batched workload, and one forward
+    model of the inversion layer; each with the GPU time of the same call
+    and the largest difference relative to max|g_z|."""
+    import harmonica_amd as hm
+    import oracle
+    from harmonica_amd.workloads import batched_models, inversion_layer, terrain_layer_small
+
+    threads = min(len(os.sched_getaffinity(0)), 16)
+    out = {"threads": threads}
+    w = terrain_layer_small()
+    layer = hm.prism_layer((w["easting"], w["northing"]), w["surface"], w["reference"],
+                           properties={"density": w["density"]})
+    bottom, top = layer.bottom.values, layer.top.values
+    pairs = w["coordinates"][0].size * w["density"].size
+    layer.prism_layer.gravity(w["coordinates"], field="g_z")  # warm-up
+    gpu, t_gpu = _sync_time(lambda: layer.prism_layer.gravity(w["coordinates"], field="g_z"))
+    t0 = time.perf_counter()
+    cpu = oracle.prism_layer_gravity(w["coordinates"], w["easting"], w["northing"], bottom, top,
+                                     w["density"], "g_z", n_threads=threads)
+    t_cpu = time.perf_counter() - t0
+    out["terrain_small_layer"] = {
+        "pairs": pairs, "gpu_s": t_gpu, "cpu_s": t_cpu, "cpu_pair_evals_per_s": pairs / t_cpu,
+        "gpu_pair_evals_per_s": pairs / t_gpu,
+        "max_rel_diff": float(np.abs(gpu - cpu).max() / np.abs(cpu).max())}
+    # 16 models spread over the batch (every 32nd of 512); the oracle runs
+    # on every 8th observation point of each, the GPU on all of them
+    models = batched_models(n_models=args.batched_models)
+    models = models[:: max(len(models) // 16, 1)][:16]
+    pairs = pairs_cpu = 0
+    t_gpu = t_cpu = 0.0
+    diff = 0.0
+    for coords, prisms, density in models:
+        hm.prism_gravity(coords, prisms, density, field="g_z")
+        g, dt = _sync_time(lambda: hm.prism_gravity(coords, prisms, density, field="g_z"))
+        t_gpu += dt
+        pairs += coords[0].size * prisms.shape[0]
+        sub = tuple(c[::8] for c in coords)
+        pairs_cpu += sub[0].size * prisms.shape[0]
+        t0 = time.perf_counter()
+        c = oracle.prism_gravity(sub, prisms, density, "g_z", n_threads=threads)
+        t_cpu += time.perf_counter() - t0
+        diff = max(diff, float(np.abs(g[::8] - c).max() / np.abs(c).max()))
+    out["batched_16_models"] = {
+        "pairs": pairs, "gpu_s": t_gpu, "gpu_pair_evals_per_s": pairs / t_gpu,
+        "cpu_pairs": pairs_cpu, "cpu_s": t_cpu, "cpu_pair_evals_per_s": pairs_cpu / t_cpu,
+        "max_rel_diff": diff, "note": "GPU: one launch per model, not the batched graph"}
+    w = inversion_layer()
+    layer = hm.prism_layer((w["easting"], w["northing"]), w["top"], w["bottom"],
+                           properties={"density": w["density"]})
+    pairs = w["coordinates"][0].size * w["density"].size
+    layer.prism_layer.gravity(w["coordinates"], field="g_z")
+    gpu, t_gpu = _sync_time(lambda: layer.prism_layer.gravity(w["coordinates"], field="g_z"))
+    t0 = time.perf_counter()
+    cpu = oracle.prism_layer_gravity(w["coordinates"], w["easting"], w["northing"],
+                                     layer.bottom.values, layer.top.values, w["density"], "g_z",
+                                     n_threads=threads)
+    t_cpu = time.perf_counter() - t0
+    out["inversion_forward"] = {
+        "pairs": pairs, "gpu_s": t_gpu, "cpu_s": t_cpu, "cpu_pair_evals_per_s": pairs / t_cpu,
+        "gpu_pair_evals_per_s": pairs / t_gpu,
+        "max_rel_diff": float(np.abs(gpu - cpu).max() / np.abs(cpu).max())}
+    for k, v in out.items():
+        if isinstance(v, dict):
+            log(f"cpu configs {k}: GPU {v['gpu_s']:.4f} s, "
+                f"CPU {v['cpu_s']:.2f} s on {threads} threads, max rel diff {v['max_rel_diff']:.2e}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -429,6 +501,8 @@ def main():
                     help="time the oracle's CPU loop on the first 3 windows")
     ap.add_argument("--eqs-oracle", action="store_true", help="check the FP64 fit on a 20k subset")
     ap.add_argument("--inversion", action="store_true", help="also run the autograd inversion")
+    ap.add_argument("--cpu-configs", action="store_true",
+                    help="also time the oracle beside the GPU on the other workloads")
     ap.add_argument("--eqs-only", choices=("float64", "mixed", "mixed-fp32"),
                     help="only run the equivalent-source fit in this precision (for profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -457,6 +531,7 @@ def main():
 
     from harmonica_amd import _lib
     from harmonica_amd._forward import ctypes_size, to_device
+    from harmonica_amd import workloads
     from harmonica_amd.workloads import layer_to_prisms, terrain_layer_large
 
     assert torch.cuda.is_available(), "no GPU"
@@ -498,7 +573,9 @@ def main():
         wall = time.perf_counter() - t0
         return ev0.elapsed_time(ev1) / steps, wall
 
-    log(f"g_z: {n_obs} points x {n_prisms} prisms, warmup {args.warmup}, steps {args.steps}")
+    input_sha256 = workloads.sha256(w)
+    log(f"g_z: {n_obs} points x {n_prisms} prisms, warmup {args.warmup}, steps {args.steps}, "
+        f"input sha256 {input_sha256[:16]}")
     step, out = make_step(_lib.FIELD_BIT["g_z"], 1)
     ms, wall = timed(step, args.steps, args.warmup)
     value = pairs / (ms * 1e-3)
@@ -542,6 +619,8 @@ def main():
         extra["eqs_gb"] = bench_eqs_gb(args)
     if args.inversion:
         extra["inversion"] = bench_inversion(args)
+    if args.cpu_configs:
+        extra["cpu_configs"] = bench_cpu_configs(args)
 
     cpu = None
     if not args.no_cpu:
@@ -566,7 +645,8 @@ def main():
         "config": {"workload": f"terrain layer {args.prism_side}x{args.prism_side} prisms "
                                f"(100 km square, bottoms -5000 m) x {args.obs_side}x{args.obs_side} "
                                "grid at 2500 m, g_z, prism_gravity path",
-                   "n_prisms": n_prisms, "n_obs": n_obs, "pairs_per_step": pairs},
+                   "n_prisms": n_prisms, "n_obs": n_obs, "pairs_per_step": pairs,
+                   "input_sha256": input_sha256},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -26,7 +26,10 @@ TENSOR_FIELDS = ("g_ee", "g_nn", "g_zz", "g_en", "g_ez", "g_nz")
 def to_device(a, dtype=torch.float64):
     """Copy a host array to a contiguous device tensor."""
     dev = _lib.require_device()
-    t = torch.as_tensor(np.ascontiguousarray(a))
+    a = np.ascontiguousarray(a)
+    if not a.flags.writeable:
+        a = a.copy()
+    t = torch.as_tensor(a)
     return t.to(device=dev, dtype=dtype).contiguous()
 
 

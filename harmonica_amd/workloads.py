@@ -24,7 +24,31 @@ reference's tests.
 * inversion_layer      200 x 200 basement layer and a 128 x 128 grid at 500 m.
 """
 
+import hashlib
+
 import numpy as np
+
+
+def sha256(workload):
+    """SHA-256 (hex) of a generated input set: every array, scalar, tuple
+    and list in ``workload`` (a dict, tuple or list, walked in key order),
+    each array as its shape followed by its FP64 little-endian bytes."""
+    h = hashlib.sha256()
+
+    def feed(name, v):
+        if isinstance(v, dict):
+            for k in sorted(v):
+                feed(f"{name}/{k}", v[k])
+        elif isinstance(v, (tuple, list)):
+            for i, item in enumerate(v):
+                feed(f"{name}/{i}", item)
+        else:
+            a = np.ascontiguousarray(np.asarray(v, dtype="<f8"))
+            h.update(f"{name}:{a.shape};".encode())
+            h.update(a.tobytes())
+
+    feed("", workload)
+    return h.hexdigest()
 
 
 def _grid(w, e, s, n, shape, height):
