@@ -321,7 +321,9 @@ def bench_eqs(args, modes=None):
     preds = {}
     small = tuple(c[:2000] for c in coords)
     if modes is None:
-        modes = ("float64", "mixed", "mixed-fp32") if args.eqs_all else ("float64", "mixed")
+        modes = ("float64", "mixed", "mixed-bf16")
+        if args.eqs_all:
+            modes += ("mixed-fp32",)
     for mode in modes:  # untimed warm-up of every code path
         hm.EquivalentSources(damping=w["damping"], depth=w["depth"], fit_precision=mode).fit(
             small, data[:2000])
@@ -503,7 +505,7 @@ def main():
     ap.add_argument("--inversion", action="store_true", help="also run the autograd inversion")
     ap.add_argument("--cpu-configs", action="store_true",
                     help="also time the oracle beside the GPU on the other workloads")
-    ap.add_argument("--eqs-only", choices=("float64", "mixed", "mixed-fp32"),
+    ap.add_argument("--eqs-only", choices=("float64", "mixed", "mixed-bf16", "mixed-fp32"),
                     help="only run the equivalent-source fit in this precision (for profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device-to-device (calibrates FETCH_SIZE/WRITE_SIZE)")

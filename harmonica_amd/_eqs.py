@@ -223,16 +223,17 @@ def fit_device(coords, data, weights, points, damping, precision="float64", max_
     the predictions falls below rtol."""
     import ctypes
 
-    modes = {"float64": _lib.HMX_FIT_FP64, "mixed": _lib.HMX_FIT_BF16_SPLIT,
-             "mixed-fp32": _lib.HMX_FIT_FP32_MFMA}
+    modes = {"float64": _lib.HMX_FIT_FP64, "mixed": _lib.HMX_FIT_INT8_SPLIT,
+             "mixed-bf16": _lib.HMX_FIT_BF16_SPLIT, "mixed-fp32": _lib.HMX_FIT_FP32_MFMA}
     mode = modes[precision]
+    split = mode in (_lib.HMX_FIT_BF16_SPLIT, _lib.HMX_FIT_INT8_SPLIT)
     if max_iter is None:
         max_iter = 10 if mode == _lib.HMX_FIT_FP64 else 300
     if rtol is None:
         rtol = 1e-12 if mode == _lib.HMX_FIT_FP64 else 1e-9
-    if mode == _lib.HMX_FIT_BF16_SPLIT:
+    if split:
         # The normal equations sum over the data, so their order is free.  In
-        # spatial order each group of 1024 data the split-bf16 normal matrix
+        # spatial order each group of 1024 data the split normal matrix
         # scales separately is a compact region, where the columns of far
         # sources are uniformly small: they keep more significant bits, and
         # the matrix preconditions better.
@@ -243,7 +244,7 @@ def fit_device(coords, data, weights, points, damping, precision="float64", max_
     p = [_dev(x, torch.float64) for x in points]
     d = _dev(data, torch.float64)
     w = None if weights is None else _dev(weights, torch.float64)
-    if mode == _lib.HMX_FIT_BF16_SPLIT:
+    if split:
         idx = torch.as_tensor(order).to(c[0].device)
         c = [x[idx].contiguous() for x in c]
         d = d[idx].contiguous()
@@ -275,8 +276,9 @@ class EquivalentSources(BaseEstimator):
     the GPU is always parallel), ``dtype`` (of the jacobian and predictions).
     ``fit_precision`` selects the GPU least-squares mode: ``"float64"``
     (default; FP64 normal matrix and Cholesky, like the reference's LAPACK
-    solve), ``"mixed"`` (normal matrix from exact bf16 MFMA products of
-    8-bit slices of the scaled jacobian, about 2^-31 relative) or
+    solve), ``"mixed"`` (normal matrix from exact int8 MFMA products of
+    8-bit slices of the scaled jacobian, about 2^-31 relative),
+    ``"mixed-bf16"`` (the same slices on bf16 MFMA, at half the rate) or
     ``"mixed-fp32"`` (FP32-MFMA normal matrix); in the mixed modes the
     FP64 factor of the low-precision matrix preconditions conjugate
     gradients on the FP64 normal equations.
@@ -307,9 +309,9 @@ class EquivalentSources(BaseEstimator):
         if depth == 0:
             msg = "Depth value cannot be zero. It should be a non-zero numeric value."
             raise ValueError(msg)
-        if fit_precision not in ("float64", "mixed", "mixed-fp32"):
-            raise ValueError(f"Invalid fit_precision '{fit_precision}': 'float64', 'mixed' or "
-                             "'mixed-fp32'.")
+        if fit_precision not in ("float64", "mixed", "mixed-bf16", "mixed-fp32"):
+            raise ValueError(f"Invalid fit_precision '{fit_precision}': 'float64', 'mixed', "
+                             "'mixed-bf16' or 'mixed-fp32'.")
         self.damping = damping
         self.points = points
         self.depth = depth

@@ -37,8 +37,8 @@ from ._eqs import (
 from ._forward import to_device
 from ._lib import check, lib, ptr, stream_handle
 
-_MODES = {"float64": _lib.HMX_FIT_FP64, "mixed": _lib.HMX_FIT_BF16_SPLIT,
-          "mixed-fp32": _lib.HMX_FIT_FP32_MFMA}
+_MODES = {"float64": _lib.HMX_FIT_FP64, "mixed": _lib.HMX_FIT_INT8_SPLIT,
+          "mixed-bf16": _lib.HMX_FIT_BF16_SPLIT, "mixed-fp32": _lib.HMX_FIT_FP32_MFMA}
 
 
 def rolling_window(coordinates, window_size, overlap, region):

@@ -82,9 +82,9 @@ class EquivalentSourcesSph(BaseEstimator):
 
     def __init__(self, damping=None, points=None, relative_depth=500, parallel=True,
                  fit_precision="float64"):
-        if fit_precision not in ("float64", "mixed", "mixed-fp32"):
-            raise ValueError(f"Invalid fit_precision '{fit_precision}': 'float64', 'mixed' or "
-                             "'mixed-fp32'.")
+        if fit_precision not in ("float64", "mixed", "mixed-bf16", "mixed-fp32"):
+            raise ValueError(f"Invalid fit_precision '{fit_precision}': 'float64', 'mixed', "
+                             "'mixed-bf16' or 'mixed-fp32'.")
         self.damping = damping
         self.points = points
         self.relative_depth = relative_depth

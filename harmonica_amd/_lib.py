@@ -29,6 +29,7 @@ HMX_SPHERICAL = 1
 HMX_FIT_FP64 = 0
 HMX_FIT_FP32_MFMA = 1
 HMX_FIT_BF16_SPLIT = 2
+HMX_FIT_INT8_SPLIT = 3
 
 #: Field names in the reference's FIELDS order (gravity.py:37-48); bit i of a
 #: field mask selects FIELDS[i].

@@ -32,6 +32,7 @@ namespace {
 typedef double double4_t __attribute__((ext_vector_type(4)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef int int32x4_t __attribute__((ext_vector_type(4)));  // also 16 packed int8
 
 constexpr int kNB = 128;    // Cholesky block size
 constexpr int kOuter = 1024;  // Cholesky outer block (K of the big trailing updates)
@@ -760,7 +761,11 @@ __global__ void zero_int_kernel(int *p) {
 }
 
 // ------------------------------------------ split-precision normal matrix
-// Exact-product scheme for X^T X on bf16 MFMA.  The rows are cut into
+// Exact-product schemes for X^T X on low-precision MFMA, in two slice
+// formats (bf16 and int8, described below) that share the row groups, the
+// group scales and the gram kernel.
+//
+// bf16 slices.  The rows are cut into
 // groups G of kFoldRows; in each group, column j of X = W^1/2 J S is scaled
 // by f_Gj = 2^(E + 1), where max_{k in G} |X_kj| < 2^E, so x = X_kj / f_Gj
 // lies in (-1/2, 1/2), and cut into kSlices integer slices:
@@ -779,10 +784,55 @@ __global__ void zero_int_kernel(int *p) {
 // largest entry.  Slices are stored k-packed: slice t, rows 8 g .. 8 g + 7
 // of column j are the 8 consecutive bf16 at ((t * n_groups + g) * m + j) * 8,
 // one 16-byte load per MFMA operand.
+//
+// int8 slices.  With the same f_Gj, x = X_kj / f_Gj in (-1/2, 1/2) is
+// rounded to the integer q = rint(c x) with c = 0xFEFEFEFE (just below
+// 2^32), so |q| <= 0x7F7F7F7F, and q is written in balanced base-256
+// digits: the bytes b_t of q + 0x80808080 (which lies in [0, 2^32)) give
+// d_t = b_t - 128 in [-128, 127], so that
+//   X_kj = (2^32 / c) f_Gj sum_t d_t,kj 256^-(t+1)   (error below f_Gj / 2c).
+// (c = 2^31 would also do, but would leave the leading digit in
+// [-64, 64] and make the dropped orders 4 times larger.)  The digits are
+// int8 exactly, products of two have at most 14 bits plus sign, and int32
+// MFMA sums are exact (one group's order sum is below 4 x 1024 x 2^14 =
+// 2^26).  The normal matrix is the bf16 formula with f_Gi f_Gj multiplied
+// by (2^32 / c)^2, a factor applied in FP64 when a group is folded.  Slice t, rows 16 g .. 16 g + 15 of column j
+// are the 16 consecutive int8 at ((t * n_units + g) * m + j) * 16 (byte q
+// of the unit is row 16 g + q), one 16-byte load per MFMA operand of
+// v_mfma_i32_16x16x64_i8, which takes twice the rows of the bf16 MFMA in
+// the same issue time.
 constexpr int kSlices = 4;
-constexpr int kFoldChunks = 32;  // 32 chunks of 32 rows: order-0 sums stay exact
-constexpr int kFoldRows = 32 * kFoldChunks;
-static_assert(kFoldChunks % 2 == 0, "the gram kernel alternates its LDS buffers in pairs of chunks");
+constexpr int kFoldRows = 1024;  // rows per group (one scale per group and column)
+
+struct SplitBf16 {
+    using unit_t = bf16x8_t;  // one slice of 8 rows of one column
+    using acc_t = float4_t;
+    static constexpr int kRowsPerUnit = 8;
+    static constexpr double kFoldScale = 1.0;  // multiplies f_Gi f_Gj
+    __device__ static acc_t mfma(unit_t a, unit_t b, acc_t c) {
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+    }
+};
+
+struct SplitInt8 {
+    using unit_t = int32x4_t;  // one slice of 16 rows of one column (16 int8)
+    using acc_t = int32x4_t;
+    static constexpr int kRowsPerUnit = 16;
+    static constexpr double kDigitScale = 4278124286.0;  // c = 0xFEFEFEFE
+    static constexpr double kFoldScale = (4294967296.0 / kDigitScale) * (4294967296.0 / kDigitScale);
+    __device__ static acc_t mfma(unit_t a, unit_t b, acc_t c) {
+        return __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, c, 0, 0, 0);
+    }
+};
+
+// A chunk of the gram kernel is 4 units (32 rows of bf16 slices, 64 of int8);
+// a group of kFoldRows rows is a whole, even number of chunks.
+template <class Tr>
+constexpr int fold_chunks() {
+    return kFoldRows / (4 * Tr::kRowsPerUnit);
+}
+static_assert(fold_chunks<SplitBf16>() % 2 == 0 && fold_chunks<SplitInt8>() % 2 == 0,
+              "the gram kernel alternates its LDS buffers in pairs of chunks");
 
 // f[G][j] = 2^(E + 1) with max_{k in group G} sqrt(w_k) J_kj / s_j < 2^E
 // (1 if all zero), for the groups G of kFoldRows rows.
@@ -865,16 +915,54 @@ __global__ __launch_bounds__(256) void slice_kernel(
     }
 }
 
+// The kSlices int8 slices of X / f (f per group of kFoldRows rows) in the
+// k-packed layout (rows >= n_data are 0).
+__global__ __launch_bounds__(256) void slice_i8_kernel(
+    const double *__restrict__ e, const double *__restrict__ n, const double *__restrict__ u,
+    const double *__restrict__ w, int64_t n_data, const double *__restrict__ pe,
+    const double *__restrict__ pn, const double *__restrict__ pu,
+    const double *__restrict__ scale, const double *__restrict__ f, int64_t n_src,
+    int64_t n_units, int32x4_t *__restrict__ S) {
+    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (j >= n_src) return;
+    const double se = pe[j], sn = pn[j], su = pu[j];
+    const double inv_s = 1.0 / scale[j];
+    for (int64_t g = blockIdx.y; g < n_units; g += gridDim.y) {
+        const double inv = inv_s / f[(16 * g / kFoldRows) * n_src + j];  // f: a power of 2
+        uint32_t word[kSlices][4] = {};
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int64_t k = 16 * g + q;
+            double x = 0.0;
+            if (k < n_data) {
+                const double sw = w ? sqrt(w[k]) : 1.0;
+                x = sw * inv_dist(e[k], n[k], u[k], se, sn, su) * inv;
+            }
+            const int32_t qi = (int32_t)rint(x * SplitInt8::kDigitScale);  // |qi| <= 0x7F7F7F7F
+            // bytes of qi + 0x80808080, each minus 128 (as int8: XOR 0x80)
+            const uint32_t d = ((uint32_t)qi + 0x80808080u) ^ 0x80808080u;
+#pragma unroll
+            for (int t = 0; t < kSlices; ++t)
+                word[t][q >> 2] |= ((d >> (8 * (kSlices - 1 - t))) & 0xffu) << (8 * (q & 3));
+        }
+#pragma unroll
+        for (int t = 0; t < kSlices; ++t)
+            S[(t * n_units + g) * n_src + j] =
+                int32x4_t{(int)word[t][0], (int)word[t][1], (int)word[t][2], (int)word[t][3]};
+    }
+}
+
 // Orders O0..O1 of the split-precision normal matrix on the upper-triangle
 // tiles: C = beta C + sum_G f_Gr f_Gc sum_o 256^-(o+2) sum_{s+t=o} S_s^T S_t
-// (+ shift on the diagonal), f the (n_fold, m) group scales.  A workgroup of 8 wavefronts owns a kST x kST
-// tile (2 x 4 waves of 64 x 32, 4 x 2 MFMA 16x16x32 bf16 tiles each) and
-// one FP32 accumulator set per order, folded into FP64 every kFoldChunks
-// chunks of 32 rows.  Slices 0..O1 of both panels go through two LDS
+// (+ shift on the diagonal), f the (n_fold, m) group scales, for the
+// slice format Tr.  A workgroup of 8 wavefronts owns a kST x kST tile
+// (2 x 4 waves of 64 x 32, 4 x 2 MFMA 16x16 tiles each) and one exact
+// accumulator set per order (FP32 for bf16 slices, int32 for int8), folded
+// into FP64 at the end of every group of kFoldRows rows.  Slices 0..O1 of both panels go through two LDS
 // buffers, loaded straight from global memory into LDS, with the loads of
 // the next chunk in flight during the MFMAs.
-template <int O0, int O1>
-__global__ __launch_bounds__(512) void gram_bf16_kernel(const bf16x8_t *__restrict__ S,
+template <class Tr, int O0, int O1>
+__global__ __launch_bounds__(512) void gram_split_kernel(const typename Tr::unit_t *__restrict__ S,
                                                         int64_t n_groups, int64_t m,
                                                         const double *__restrict__ f,
                                                         double *__restrict__ C, int64_t ldc,
@@ -882,17 +970,20 @@ __global__ __launch_bounds__(512) void gram_bf16_kernel(const bf16x8_t *__restri
                                                         int64_t tiles_n) {
     constexpr int kNO = O1 - O0 + 1;  // orders accumulated here
     constexpr int kNS = O1 + 1;       // slices needed on each side
+    constexpr int kFoldChunks = fold_chunks<Tr>();
+    using unit_t = typename Tr::unit_t;
+    using acc_t = typename Tr::acc_t;
     // Two staging buffers as separate arrays, so that the compiler can tell
     // the buffer being filled from the one being read.
-    __shared__ bf16x8_t lds0[2][kNS][4][kST];
-    __shared__ bf16x8_t lds1[2][kNS][4][kST];
+    __shared__ unit_t lds0[2][kNS][4][kST];
+    __shared__ unit_t lds1[2][kNS][4][kST];
     int64_t rb, cb;
     if (!upper_tile_xcd(blockIdx.x, tiles_n, rb, cb)) return;
     const int64_t r0 = rb * kST, c0 = cb * kST;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wr = wave >> 2, wc = wave & 3;
-    const bf16x8_t zero8 = {};
-    float4_t acc[kNO][4][2];
+    const unit_t zero_unit = {};
+    acc_t acc[kNO][4][2];
     double4_t dacc[4][2];
 #pragma unroll
     for (int ti = 0; ti < 4; ++ti)
@@ -905,7 +996,7 @@ __global__ __launch_bounds__(512) void gram_bf16_kernel(const bf16x8_t *__restri
     // the products they enter belong to output columns past m, never stored.
     const int kg = threadIdx.x >> 7, col = threadIdx.x & (kST - 1);
     const int col0 = col & ~63;
-    auto issue = [&](int ch, bf16x8_t (&L)[2][kNS][4][kST]) {
+    auto issue = [&](int ch, unit_t (&L)[2][kNS][4][kST]) {
         const int64_t g = 4 * (int64_t)ch + kg;
 #pragma unroll
         for (int side = 0; side < 2; ++side) {
@@ -919,13 +1010,13 @@ __global__ __launch_bounds__(512) void gram_bf16_kernel(const bf16x8_t *__restri
         }
     };
     // MFMAs of one chunk from the staged buffer L.
-    auto compute = [&](const bf16x8_t (&L)[2][kNS][4][kST]) {
+    auto compute = [&](const unit_t (&L)[2][kNS][4][kST]) {
 #pragma unroll
         for (int o = 0; o < kNO; ++o)
 #pragma unroll
             for (int sa = 0; sa <= O0 + o; ++sa) {
                 const int sb = O0 + o - sa;
-                bf16x8_t a[4], bb[2];
+                unit_t a[4], bb[2];
 #pragma unroll
                 for (int t = 0; t < 4; ++t) a[t] = L[0][sa][lane >> 4][64 * wr + 16 * t + (lane & 15)];
 #pragma unroll
@@ -934,8 +1025,7 @@ __global__ __launch_bounds__(512) void gram_bf16_kernel(const bf16x8_t *__restri
                 for (int ti = 0; ti < 4; ++ti)
 #pragma unroll
                     for (int tj = 0; tj < 2; ++tj)
-                        acc[o][ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                            a[ti], bb[tj], acc[o][ti][tj], 0, 0, 0);
+                        acc[o][ti][tj] = Tr::mfma(a[ti], bb[tj], acc[o][ti][tj]);
             }
     };
     // The chunk's loads complete (this wave's), then every wave's.
@@ -955,8 +1045,7 @@ __global__ __launch_bounds__(512) void gram_bf16_kernel(const bf16x8_t *__restri
             for (int ti = 0; ti < 4; ++ti)
 #pragma unroll
                 for (int tj = 0; tj < 2; ++tj)
-                    acc[o][ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                        zero8, zero8, float4_t{0, 0, 0, 0}, 0, 0, 0);
+                    acc[o][ti][tj] = Tr::mfma(zero_unit, zero_unit, acc_t{0, 0, 0, 0});
         // kFoldChunks is even, so every fold starts in lds0; pairs of
         // chunks alternate lds0 and lds1 with the buffer known statically.
         int ch = g0;
@@ -993,7 +1082,7 @@ __global__ __launch_bounds__(512) void gram_bf16_kernel(const bf16x8_t *__restri
 #pragma unroll
                     for (int o = 0; o < kNO; ++o)
                         sum = fma((double)acc[o][ti][tj][v], ldexp(1.0, -8 * (O0 + o + 2)), sum);
-                    dacc[ti][tj][v] = fma(sum, fr * fc[tj], dacc[ti][tj][v]);
+                    dacc[ti][tj][v] = fma(sum, fr * fc[tj] * Tr::kFoldScale, dacc[ti][tj][v]);
                 }
             }
     }
@@ -1259,15 +1348,20 @@ int hmx_eqs_fit_workspace_bytes(int64_t n_data, int64_t n_src, int precision_mod
     if (!bytes) return fail(HMX_ERR_ARG, "bytes is NULL");
     if (n_data <= 0 || n_src <= 0) return fail(HMX_ERR_ARG, "empty problem");
     if (precision_mode != HMX_FIT_FP64 && precision_mode != HMX_FIT_FP32_MFMA &&
-        precision_mode != HMX_FIT_BF16_SPLIT)
+        precision_mode != HMX_FIT_BF16_SPLIT && precision_mode != HMX_FIT_INT8_SPLIT)
         return fail(HMX_ERR_ARG, "invalid precision mode %d", precision_mode);
     const size_t m = (size_t)n_src, n = (size_t)n_data;
     const size_t nblk = (m + kNB - 1) / kNB;
-    const size_t n_pad = (n + 31) / 32 * 32;
     size_t x_bytes = n * m * sizeof(double);                        // X = W^1/2 J S
     if (precision_mode == HMX_FIT_FP32_MFMA) x_bytes = n * m * sizeof(float);
-    if (precision_mode == HMX_FIT_BF16_SPLIT)  // slices and the group scales
-        x_bytes = kSlices * n_pad * m * 2 + (n_pad + kFoldRows - 1) / kFoldRows * m * sizeof(double);
+    if (precision_mode == HMX_FIT_BF16_SPLIT || precision_mode == HMX_FIT_INT8_SPLIT) {
+        // slices (16-byte units of 8 or 16 rows, padded to whole chunks of 4
+        // units) and the group scales
+        const size_t rows = precision_mode == HMX_FIT_INT8_SPLIT ? 16 : 8;
+        const size_t n_units = (n + 4 * rows - 1) / (4 * rows) * 4;
+        x_bytes = kSlices * n_units * m * 16 +
+                  (n_units * rows + kFoldRows - 1) / kFoldRows * m * sizeof(double);
+    }
     const size_t copies = precision_mode == HMX_FIT_FP64 ? 1 : 2;
     *bytes = copies * m * m * sizeof(double)             // A, then its factor U (+ a copy of A)
              + x_bytes + 2 * nblk * kNB * kNB * sizeof(double)  // inverses of diagonal blocks
@@ -1299,9 +1393,13 @@ int hmx_eqs_fit(const double *easting, const double *northing, const double *upw
     const int64_t m = n_src, nd = n_data;
     // mixed: a low-precision normal matrix preconditions CG on the FP64 one
     const bool mixed = precision_mode != HMX_FIT_FP64;
-    const bool split = precision_mode == HMX_FIT_BF16_SPLIT;
+    const bool int8 = precision_mode == HMX_FIT_INT8_SPLIT;
+    const bool split = precision_mode == HMX_FIT_BF16_SPLIT || int8;
     const size_t nblk = (size_t)((m + kNB - 1) / kNB);
-    const int64_t n_groups = (nd + 31) / 32 * 4;  // k-groups of 8 rows, padded to 32 rows
+    // split modes: 16-byte slice units of 8 (bf16) or 16 (int8) rows, padded
+    // to whole chunks of 4 units
+    const int64_t unit_rows = int8 ? 16 : 8;
+    const int64_t n_groups = (nd + 4 * unit_rows - 1) / (4 * unit_rows) * 4;
 
     char *ws = static_cast<char *>(workspace);
     Fit f;
@@ -1325,9 +1423,9 @@ int hmx_eqs_fit(const double *easting, const double *northing, const double *upw
     }
     void *X = ws;
     double *fgroup = nullptr;  // split: (n_fold, m) power-of-2 column scales per row group
-    const int64_t n_fold = (n_groups * 8 + kFoldRows - 1) / kFoldRows;
+    const int64_t n_fold = (n_groups * unit_rows + kFoldRows - 1) / kFoldRows;
     if (split) {
-        ws += (size_t)kSlices * n_groups * 8 * m * 2;
+        ws += (size_t)kSlices * n_groups * m * 16;
         fgroup = reinterpret_cast<double *>(ws);
         ws += (size_t)n_fold * m * sizeof(double);
     } else
@@ -1360,10 +1458,17 @@ int hmx_eqs_fit(const double *easting, const double *northing, const double *upw
                            fgroup);
         HMX_LAUNCHED("colscale_pow2_kernel");
         const unsigned gg = (unsigned)(n_groups < 4096 ? n_groups : 4096);
-        hipLaunchKernelGGL(slice_kernel, dim3(gm, gg), dim3(256), 0, s, easting, northing, upward,
-                           weights, nd, p_easting, p_northing, p_upward, f.scale, fgroup, m,
-                           n_groups, static_cast<bf16x8_t *>(X));
-        HMX_LAUNCHED("slice_kernel");
+        if (int8) {
+            hipLaunchKernelGGL(slice_i8_kernel, dim3(gm, gg), dim3(256), 0, s, easting, northing,
+                               upward, weights, nd, p_easting, p_northing, p_upward, f.scale,
+                               fgroup, m, n_groups, static_cast<int32x4_t *>(X));
+            HMX_LAUNCHED("slice_i8_kernel");
+        } else {
+            hipLaunchKernelGGL(slice_kernel, dim3(gm, gg), dim3(256), 0, s, easting, northing,
+                               upward, weights, nd, p_easting, p_northing, p_upward, f.scale,
+                               fgroup, m, n_groups, static_cast<bf16x8_t *>(X));
+            HMX_LAUNCHED("slice_kernel");
+        }
     } else if (mixed) {
         hipLaunchKernelGGL(scaled_jacobian_kernel<float>, dim3(gm, gy), dim3(256), 0, s, easting,
                            northing, upward, weights, nd, p_easting, p_northing, p_upward, f.scale,
@@ -1406,13 +1511,27 @@ int hmx_eqs_fit(const double *easting, const double *northing, const double *upw
             if (split) {
                 const int64_t tiles = (m + kST - 1) / kST;
                 const unsigned blocks = upper_tile_xcd_blocks(tiles);
-                const bf16x8_t *Sl = static_cast<const bf16x8_t *>(X);
-                hipLaunchKernelGGL((gram_bf16_kernel<0, 1>), dim3(blocks), dim3(512), 0, s, Sl,
-                                   n_groups, m, fgroup, f.A, m, 0.0, damping, tiles);
-                HMX_LAUNCHED("gram_bf16_kernel<0,1>");
-                hipLaunchKernelGGL((gram_bf16_kernel<2, 3>), dim3(blocks), dim3(512), 0, s, Sl,
-                                   n_groups, m, fgroup, f.A, m, 1.0, 0.0, tiles);
-                HMX_LAUNCHED("gram_bf16_kernel<2,3>");
+                if (int8) {
+                    const int32x4_t *Sl = static_cast<const int32x4_t *>(X);
+                    hipLaunchKernelGGL((gram_split_kernel<SplitInt8, 0, 1>), dim3(blocks),
+                                       dim3(512), 0, s, Sl, n_groups, m, fgroup, f.A, m, 0.0,
+                                       damping, tiles);
+                    HMX_LAUNCHED("gram_split_kernel<SplitInt8,0,1>");
+                    hipLaunchKernelGGL((gram_split_kernel<SplitInt8, 2, 3>), dim3(blocks),
+                                       dim3(512), 0, s, Sl, n_groups, m, fgroup, f.A, m, 1.0, 0.0,
+                                       tiles);
+                    HMX_LAUNCHED("gram_split_kernel<SplitInt8,2,3>");
+                } else {
+                    const bf16x8_t *Sl = static_cast<const bf16x8_t *>(X);
+                    hipLaunchKernelGGL((gram_split_kernel<SplitBf16, 0, 1>), dim3(blocks),
+                                       dim3(512), 0, s, Sl, n_groups, m, fgroup, f.A, m, 0.0,
+                                       damping, tiles);
+                    HMX_LAUNCHED("gram_split_kernel<SplitBf16,0,1>");
+                    hipLaunchKernelGGL((gram_split_kernel<SplitBf16, 2, 3>), dim3(blocks),
+                                       dim3(512), 0, s, Sl, n_groups, m, fgroup, f.A, m, 1.0, 0.0,
+                                       tiles);
+                    HMX_LAUNCHED("gram_split_kernel<SplitBf16,2,3>");
+                }
             } else {
                 st = launch_syrk<float>(static_cast<const float *>(X), m, nd, m, f.A, m, 0.0, 1.0,
                                         damping, s);

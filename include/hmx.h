@@ -75,6 +75,8 @@ extern "C" {
 #define HMX_FIT_FP32_MFMA 1 /* J^T W J on FP32 MFMA, FP64 Cholesky, FP64-residual CG */
 #define HMX_FIT_BF16_SPLIT 2 /* J^T W J from exact bf16 slice products on MFMA, FP64
                                 Cholesky, FP64-residual CG */
+#define HMX_FIT_INT8_SPLIT 3 /* J^T W J from exact int8 slice products on int8 MFMA,
+                                FP64 Cholesky, FP64-residual CG */
 
 /* Last error message of the calling thread ("" if none). */
 const char *hmx_last_error(void);
@@ -305,10 +307,12 @@ int hmx_eqs_fit_workspace_bytes(int64_t n_data, int64_t n_src, int precision_mod
  *   data, weights (may be NULL): (n_data) device arrays;
  *   precision_mode: HMX_FIT_FP64 (FP64 MFMA normal matrix, FP64 Cholesky,
  *     direct solve), HMX_FIT_FP32_MFMA (FP32 Jacobian and FP32-MFMA normal
- *     matrix) or HMX_FIT_BF16_SPLIT (the scaled Jacobian cut into four
- *     8-bit integer slices per column, normal matrix from the exact bf16
- *     MFMA products of the slice pairs up to order 3, about 2^-31 of its
- *     entries); in both low-precision modes the normal matrix, shifted if
+ *     matrix), HMX_FIT_BF16_SPLIT (the scaled Jacobian cut into four
+ *     8-bit integer slices per column and group of 1024 rows, normal matrix
+ *     from the exact bf16 MFMA products of the slice pairs up to order 3,
+ *     about 2^-31 of its entries) or HMX_FIT_INT8_SPLIT (the same with
+ *     balanced int8 digits on int8 MFMA, also about 2^-31); in the low-precision
+ *     modes the normal matrix, shifted if
  *     needed, is factorised in FP64 and used as the preconditioner of
  *     conjugate gradients on the exact FP64 normal equations;
  *   max_iter, rtol: CG limit and stop rule (relative change of the

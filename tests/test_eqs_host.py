@@ -113,3 +113,35 @@ def test_get_params_roundtrip():
     assert params["damping"] == 0.1 and params["fit_precision"] == "mixed"
     with pytest.raises(ValueError, match="fit_precision"):
         EquivalentSources(fit_precision="half")
+
+
+def test_fit_workspace_bytes_per_mode():
+    """hmx_eqs_fit_workspace_bytes (host arithmetic only): the FP64 mode
+    holds the FP64 scaled jacobian, the FP32 mode an FP32 one plus a copy of
+    the normal matrix, the split modes four slices per entry (2 bytes each
+    for bf16, 1 for int8, rows padded to 32 or 64) and one scale per column
+    and group of 1024 rows; unknown modes are rejected."""
+    import ctypes
+
+    from harmonica_amd import _lib
+
+    L = _lib.lib()
+
+    def nbytes(n, m, mode):
+        b = ctypes.c_size_t(0)
+        st = L.hmx_eqs_fit_workspace_bytes(n, m, mode, ctypes.byref(b))
+        return st, b.value
+
+    n, m = 5000, 700
+    st, b64 = nbytes(n, m, _lib.HMX_FIT_FP64)
+    assert st == _lib.HMX_OK
+    common = b64 - n * m * 8
+    st, b32 = nbytes(n, m, _lib.HMX_FIT_FP32_MFMA)
+    assert st == 0 and b32 == common + m * m * 8 + n * m * 4
+    n_fold = 5
+    st, bb = nbytes(n, m, _lib.HMX_FIT_BF16_SPLIT)
+    assert st == 0 and bb == common + m * m * 8 + 4 * 5024 * m * 2 + n_fold * m * 8
+    st, bi = nbytes(n, m, _lib.HMX_FIT_INT8_SPLIT)
+    assert st == 0 and bi == common + m * m * 8 + 4 * 5056 * m + n_fold * m * 8
+    assert nbytes(n, m, 4)[0] == _lib.HMX_ERR_ARG
+    assert nbytes(0, m, _lib.HMX_FIT_FP64)[0] == _lib.HMX_ERR_ARG

@@ -224,11 +224,12 @@ def test_default_depth_fit(hm, synthetic):
     npt.assert_allclose(eqs.depth_, 4.5 * spacing)
 
 
-@pytest.mark.parametrize("precision", ["mixed", "mixed-fp32"])
+@pytest.mark.parametrize("precision", ["mixed", "mixed-bf16", "mixed-fp32"])
 @pytest.mark.parametrize("damping", [0.1, 1e-3])
 def test_mixed_precision_fit_matches_fp64(hm, damping, precision):
-    """Preconditioned mixed-precision fits (bf16 slice products, or FP32 MFMA
-    with FP64 sums; FP64 Cholesky and FP64 residuals) against the FP64 fit:
+    """Preconditioned mixed-precision fits (int8 or bf16 slice products, or
+    FP32 MFMA with FP64 sums; FP64 Cholesky and FP64 residuals) against the
+    FP64 fit:
     predictions on a grid agree to 1e-6 of their maximum, also at damping
     1e-3 where the normal matrix has cond ~ 1e11 (there the FP32 normal
     matrix needs a shift and the refinement takes tens of iterations)."""
@@ -244,5 +245,31 @@ def test_mixed_precision_fit_matches_fp64(hm, damping, precision):
     p64, pmx = fp64.predict(grid), mixed.predict(grid)
     err = np.abs(pmx - p64).max() / np.abs(p64).max()
     assert err <= 1e-6, (err, mixed.fit_info_)
-    if precision == "mixed":  # the split normal matrix is close to the FP64 one
+    if precision != "mixed-fp32":  # the split normal matrix is close to the FP64 one
         assert mixed.fit_info_["iterations"] <= 30, mixed.fit_info_
+
+
+@pytest.mark.parametrize("precision", ["mixed", "mixed-bf16"])
+@pytest.mark.parametrize("n_data", [5000, 2049])
+def test_split_normal_matrix_direct_solve(hm, precision, n_data):
+    """The split normal matrix on its own: with no refinement (max_iter 0)
+    the coefficients come from the direct solve with the low-precision
+    matrix, which on a well-conditioned system (damping 1e3 on columns
+    scaled to unit variance: cond below ~1e4) must agree with the FP64
+    direct solve to about cond x 2^-31.  A wrong slice, digit or operand
+    layout in the gram kernel gives differences of order 1.  700 sources
+    (6 column tiles, the last partial); 5000 or 2049 data (several row
+    groups of 1024, the last partial, odd chunk counts)."""
+    from harmonica_amd._eqs import fit_device
+
+    rng = np.random.default_rng(21)
+    L = 40e3
+    coords = (rng.uniform(0, L, n_data), rng.uniform(0, L, n_data), rng.uniform(500, 1500, n_data))
+    pts = (rng.uniform(0, L, 700), rng.uniform(0, L, 700), rng.uniform(-3000, -1000, 700))
+    data = rng.normal(0, 1, n_data)
+    w = rng.uniform(0.5, 2.0, n_data)
+    c64, _ = fit_device(coords, data, w, pts, 1e3, "float64", max_iter=0)
+    cmx, info = fit_device(coords, data, w, pts, 1e3, precision, max_iter=0)
+    assert info["shift"] == 0.0, info
+    err = np.abs(cmx - c64).max() / np.abs(c64).max()
+    assert err <= 1e-5, err
