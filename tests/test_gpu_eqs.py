@@ -250,8 +250,8 @@ def test_mixed_precision_fit_matches_fp64(hm, damping, precision):
 
 
 @pytest.mark.parametrize("precision", ["mixed", "mixed-bf16"])
-@pytest.mark.parametrize("n_data", [5000, 2049])
-def test_split_normal_matrix_direct_solve(hm, precision, n_data):
+@pytest.mark.parametrize("n_data,n_src", [(5000, 700), (2049, 700), (6000, 2500)])
+def test_split_normal_matrix_direct_solve(hm, precision, n_data, n_src):
     """The split normal matrix on its own: with no refinement (max_iter 0)
     the coefficients come from the direct solve with the low-precision
     matrix, which on a well-conditioned system (damping 1e3 on columns
@@ -259,13 +259,15 @@ def test_split_normal_matrix_direct_solve(hm, precision, n_data):
     direct solve to about cond x 2^-31.  A wrong slice, digit or operand
     layout in the gram kernel gives differences of order 1.  700 sources
     (6 column tiles, the last partial); 5000 or 2049 data (several row
-    groups of 1024, the last partial, odd chunk counts)."""
+    groups of 1024, the last partial, odd chunk counts).  With 2500
+    sources the factorisation makes two rank-1024 trailing updates (1476
+    and 452 columns)."""
     from harmonica_amd._eqs import fit_device
 
     rng = np.random.default_rng(21)
     L = 40e3
     coords = (rng.uniform(0, L, n_data), rng.uniform(0, L, n_data), rng.uniform(500, 1500, n_data))
-    pts = (rng.uniform(0, L, 700), rng.uniform(0, L, 700), rng.uniform(-3000, -1000, 700))
+    pts = (rng.uniform(0, L, n_src), rng.uniform(0, L, n_src), rng.uniform(-3000, -1000, n_src))
     data = rng.normal(0, 1, n_data)
     w = rng.uniform(0.5, 2.0, n_data)
     c64, _ = fit_device(coords, data, w, pts, 1e3, "float64", max_iter=0)
