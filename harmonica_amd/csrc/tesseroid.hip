@@ -29,15 +29,43 @@ constexpr double kGT = 6.6743e-11;  // harmonica/constants.py:12
 // numpy.polynomial.legendre.leggauss(2): nodes -+this, weights 1.
 constexpr double kGlqNode = 0x1.279a74590331cp-1;
 
-struct alignas(16) StagedTess {
-    double t[6];
+// Constants of one undivided tesseroid, computed once per tesseroid by
+// tess_root_kernel and staged in LDS: the box and the densities at the
+// radial nodes, the dimensions and centre of the discretisation test, and
+// the trigonometry, radii and masses of the 2 x 2 x 2 quadrature nodes.
+// Most pairs never divide the tesseroid, so these replace the ~18
+// trigonometric evaluations per pair of the generic path by products with
+// the point's own sines and cosines (cos(a - b) = cos a cos b + sin a sin b).
+struct alignas(16) TessRoot {
+    double t[6];    // w, e, s, n (degrees), bottom, top (m)
     double rho[2];  // density at the lower and upper radial node
+    double l_lon, l_lat, l_rad;
+    double sin_lat_c, cos_lat_c, sin_lon_c, cos_lon_c, r_c;
+    double sin_lat_n[2], cos_lat_n[2], sin_lon_n[2], cos_lon_n[2], radius_n[2];
+    double mass[2][2];  // [latitude node][radial node]: rho a r^2 cos(lat)
 };
+constexpr int kRootDoubles = sizeof(TessRoot) / sizeof(double);
+static_assert(sizeof(TessRoot) == 30 * sizeof(double), "TessRoot is 30 doubles");
 
 struct TessPoint {
     double lon, lat, r;            // degrees, degrees, m
     double lon_rad, cosphi, sinphi;
+    double coslon, sinlon;         // of lon_rad
 };
+
+// _tesseroid_dimensions (_tesseroid_utils.py): the longitudinal,
+// latitudinal and radial sizes of a tesseroid at its top.
+__device__ __forceinline__ void tess_dimensions(const double (&t)[6], double &l_lon,
+                                                double &l_lat, double &l_rad) {
+    const double w = t[0] * kDegT, e = t[1] * kDegT, s = t[2] * kDegT, n = t[3] * kDegT;
+    double sn, cn, ss, cs, sc, cc;
+    sincos(n, &sn, &cn);
+    sincos(s, &ss, &cs);
+    sincos((n + s) / 2, &sc, &cc);
+    l_lat = t[5] * acos(sn * ss + cn * cs);
+    l_lon = t[5] * acos(sc * sc + cc * cc * cos(e - w));
+    l_rad = t[5] - t[4];
+}
 
 // distance_spherical (_forward/utils.py:121-201) from precomputed point
 // quantities to (lon_p [deg], lat_p [deg], r_p).
@@ -91,72 +119,151 @@ __device__ __forceinline__ double tess_glq(const TessPoint &P, const double (&t)
     return result;
 }
 
-// One (point, tesseroid) pair: adaptive discretisation with the lane's
-// stack, small tesseroids integrated in the order they are produced.
-template <bool kPot, bool kRadial>
-__device__ double tess_pair(const TessPoint &P, const StagedTess &T, double ratio, int *ovf) {
-    double stk[kTessStack][6];
-    int top = 0;
-    long n_small = 0;
+// tess_glq of an undivided tesseroid from its precomputed constants (the
+// same nodes, masses and order of summation).
+template <bool kPot>
+__device__ __forceinline__ double tess_glq_root(const TessPoint &P, const TessRoot &R) {
+    double coslambda[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) coslambda[i] = R.cos_lon_n[i] * P.coslon + R.sin_lon_n[i] * P.sinlon;
     double result = 0.0;
 #pragma unroll
-    for (int c = 0; c < 6; ++c) stk[0][c] = T.t[c];
-    while (top >= 0) {
-        double t[6];
+    for (int j = 0; j < 2; ++j)
 #pragma unroll
-        for (int c = 0; c < 6; ++c) t[c] = stk[top][c];
-        --top;
-        // _tesseroid_dimensions
-        const double w = t[0] * kDegT, e = t[1] * kDegT, s = t[2] * kDegT, n = t[3] * kDegT;
-        double sn, cn, ss, cs, sc, cc;
-        sincos(n, &sn, &cn);
-        sincos(s, &ss, &cs);
-        sincos((n + s) / 2, &sc, &cc);
-        const double l_lat = t[5] * acos(sn * ss + cn * cs);
-        const double l_lon = t[5] * acos(sc * sc + cc * cc * cos(e - w));
-        const double l_rad = t[5] - t[4];
-        // _distance_tesseroid_point
-        const double d = tess_distance(P, (t[0] + t[1]) / 2, (t[2] + t[3]) / 2, (t[4] + t[5]) / 2);
-        const int n_lon = d / l_lon < ratio ? 2 : 1;
-        const int n_lat = d / l_lat < ratio ? 2 : 1;
-        const int n_rad = (kRadial && d / l_rad < ratio) ? 2 : 1;
-        if (n_lon * n_lat * n_rad > 1) {
-            if ((top + 1) + n_lon * n_lat * n_rad > kTessStack) {
-                *ovf = 1;
-                return 0.0;
+        for (int k = 0; k < 2; ++k) {
+            const double radius_p = R.radius_n[k];
+            const double dr = P.r - radius_p;
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                const double cospsi = R.sin_lat_n[j] * P.sinphi + R.cos_lat_n[j] * P.cosphi * coslambda[i];
+                const double dist = sqrt(dr * dr + 2.0 * P.r * radius_p * (1.0 - cospsi));
+                double kv;
+                if constexpr (kPot) {
+                    kv = 1.0 / dist * kGT;
+                } else {
+                    const double dz = P.r - radius_p * cospsi;
+                    kv = -kGT * dz / (dist * dist * dist);
+                }
+                result += R.mass[j][k] * kv;
             }
-            const double d_lon = (t[1] - t[0]) / n_lon, d_lat = (t[3] - t[2]) / n_lat;
-            const double d_rad = (t[5] - t[4]) / n_rad;
-            for (int i = 0; i < n_lon; ++i)
-                for (int j = 0; j < n_lat; ++j)
-                    for (int k = 0; k < n_rad; ++k) {
-                        ++top;
-                        stk[top][0] = t[0] + d_lon * i;
-                        stk[top][1] = t[0] + d_lon * (i + 1);
-                        stk[top][2] = t[2] + d_lat * j;
-                        stk[top][3] = t[2] + d_lat * (j + 1);
-                        stk[top][4] = t[4] + d_rad * k;
-                        stk[top][5] = t[4] + d_rad * (k + 1);
-                    }
-        } else {
-            if (n_small + 1 > kTessMaxSmall) {
-                *ovf = 2;
-                return 0.0;
-            }
-            ++n_small;
-            result += tess_glq<kPot>(P, t, T.rho);
         }
-    }
     return result;
+}
+
+// The parts of tesseroid t for a split into n_lon x n_lat x n_rad, pushed
+// on the stack (false if the stack would overflow).
+__device__ __forceinline__ bool tess_push_parts(double (&stk)[kTessStack][6], int &top,
+                                                const double (&t)[6], int n_lon, int n_lat,
+                                                int n_rad) {
+    if ((top + 1) + n_lon * n_lat * n_rad > kTessStack) return false;
+    const double d_lon = (t[1] - t[0]) / n_lon, d_lat = (t[3] - t[2]) / n_lat;
+    const double d_rad = (t[5] - t[4]) / n_rad;
+    for (int i = 0; i < n_lon; ++i)
+        for (int j = 0; j < n_lat; ++j)
+            for (int k = 0; k < n_rad; ++k) {
+                ++top;
+                stk[top][0] = t[0] + d_lon * i;
+                stk[top][1] = t[0] + d_lon * (i + 1);
+                stk[top][2] = t[2] + d_lat * j;
+                stk[top][3] = t[2] + d_lat * (j + 1);
+                stk[top][4] = t[4] + d_rad * k;
+                stk[top][5] = t[4] + d_rad * (k + 1);
+            }
+    return true;
+}
+
+// One (point, tesseroid) pair: adaptive discretisation with the lane's
+// stack, small tesseroids integrated in the order they are produced.  The
+// undivided tesseroid is tested and, if small enough, integrated from its
+// precomputed constants; its parts go through the generic path.
+template <bool kPot, bool kRadial>
+__device__ double tess_pair(const TessPoint &P, const TessRoot &R, double ratio, int *ovf) {
+    {
+        // _distance_tesseroid_point from the centre
+        const double coslambda = R.cos_lon_c * P.coslon + R.sin_lon_c * P.sinlon;
+        const double cospsi = R.sin_lat_c * P.sinphi + R.cos_lat_c * P.cosphi * coslambda;
+        const double dr = P.r - R.r_c;
+        const double d = sqrt(dr * dr + 2.0 * P.r * R.r_c * (1.0 - cospsi));
+        const int n_lon = d / R.l_lon < ratio ? 2 : 1;
+        const int n_lat = d / R.l_lat < ratio ? 2 : 1;
+        const int n_rad = (kRadial && d / R.l_rad < ratio) ? 2 : 1;
+        if (n_lon * n_lat * n_rad == 1) return tess_glq_root<kPot>(P, R);
+        // divided: continue below with the parts
+        double stk[kTessStack][6];
+        int top = -1;
+        (void)tess_push_parts(stk, top, R.t, n_lon, n_lat, n_rad);  // at most 8 < kTessStack
+        long n_small = 0;
+        double result = 0.0;
+        while (top >= 0) {
+            double t[6];
+#pragma unroll
+            for (int c = 0; c < 6; ++c) t[c] = stk[top][c];
+            --top;
+            double l_lon, l_lat, l_rad;
+            tess_dimensions(t, l_lon, l_lat, l_rad);
+            // _distance_tesseroid_point
+            const double d2 =
+                tess_distance(P, (t[0] + t[1]) / 2, (t[2] + t[3]) / 2, (t[4] + t[5]) / 2);
+            const int m_lon = d2 / l_lon < ratio ? 2 : 1;
+            const int m_lat = d2 / l_lat < ratio ? 2 : 1;
+            const int m_rad = (kRadial && d2 / l_rad < ratio) ? 2 : 1;
+            if (m_lon * m_lat * m_rad > 1) {
+                if (!tess_push_parts(stk, top, t, m_lon, m_lat, m_rad)) {
+                    *ovf = 1;
+                    return 0.0;
+                }
+            } else {
+                if (n_small + 1 > kTessMaxSmall) {
+                    *ovf = 2;
+                    return 0.0;
+                }
+                ++n_small;
+                result += tess_glq<kPot>(P, t, R.rho);
+            }
+        }
+        return result;
+    }
+}
+
+// The TessRoot constants of each tesseroid (one per thread).
+__global__ __launch_bounds__(256) void tess_root_kernel(const double *__restrict__ tess,
+                                                        const double *__restrict__ rho,
+                                                        int64_t n, TessRoot *__restrict__ roots) {
+    const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (q >= n) return;
+    TessRoot R;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) R.t[c] = tess[6 * q + c];
+    R.rho[0] = rho[2 * q];
+    R.rho[1] = rho[2 * q + 1];
+    tess_dimensions(R.t, R.l_lon, R.l_lat, R.l_rad);
+    const double w = R.t[0], e = R.t[1], s = R.t[2], n_ = R.t[3], bottom = R.t[4], top = R.t[5];
+    sincos((s + n_) / 2 * kDegT, &R.sin_lat_c, &R.cos_lat_c);
+    sincos((w + e) / 2 * kDegT, &R.sin_lon_c, &R.cos_lon_c);
+    R.r_c = (bottom + top) / 2;
+    const double a_factor = 1.0 / 8.0 * ((e - w) * kDegT) * ((n_ - s) * kDegT) * (top - bottom);
+    const double node[2] = {-kGlqNode, kGlqNode};
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        sincos((0.5 * (n_ - s) * node[j] + 0.5 * (n_ + s)) * kDegT, &R.sin_lat_n[j], &R.cos_lat_n[j]);
+        sincos((0.5 * (e - w) * node[j] + 0.5 * (e + w)) * kDegT, &R.sin_lon_n[j], &R.cos_lon_n[j]);
+        R.radius_n[j] = 0.5 * (top - bottom) * node[j] + 0.5 * (top + bottom);
+    }
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int k = 0; k < 2; ++k)
+            R.mass[j][k] = R.rho[k] * a_factor * (R.radius_n[k] * R.radius_n[k] * R.cos_lat_n[j]);
+    roots[q] = R;
 }
 
 template <bool kPot, bool kRadial>
 __global__ __launch_bounds__(kObsPerBlock) void tesseroid_kernel(
     const double *__restrict__ lon, const double *__restrict__ lat,
-    const double *__restrict__ rad, int n_obs, const double *__restrict__ tess,
-    const double *__restrict__ rho, int64_t n_src, int64_t chunk_len, double ratio,
-    double *__restrict__ partial, int *__restrict__ overflow) {
-    __shared__ StagedTess tile[kTile];
+    const double *__restrict__ rad, int n_obs, const TessRoot *__restrict__ roots,
+    int64_t n_src, int64_t chunk_len, double ratio, double *__restrict__ partial,
+    int *__restrict__ overflow) {
+    __shared__ TessRoot tile[kTile];
     const int i = blockIdx.x * kObsPerBlock + threadIdx.x;
     const bool valid = i < n_obs;
     const int ii = valid ? i : n_obs - 1;
@@ -166,6 +273,7 @@ __global__ __launch_bounds__(kObsPerBlock) void tesseroid_kernel(
     P.r = rad[ii];
     P.lon_rad = P.lon * kDegT;
     sincos(P.lat * kDegT, &P.sinphi, &P.cosphi);
+    sincos(P.lon_rad, &P.sinlon, &P.coslon);
     double acc = 0.0;
     int ovf = 0;
     const int64_t c0 = (int64_t)blockIdx.y * chunk_len;
@@ -173,14 +281,10 @@ __global__ __launch_bounds__(kObsPerBlock) void tesseroid_kernel(
     for (int64_t t0 = c0; t0 < c1; t0 += kTile) {
         const int cnt = (int)(c1 - t0 < kTile ? c1 - t0 : kTile);
         __syncthreads();
-        for (int q = threadIdx.x; q < cnt; q += blockDim.x) {
-            StagedTess st;
-#pragma unroll
-            for (int c = 0; c < 6; ++c) st.t[c] = tess[6 * (t0 + q) + c];
-            st.rho[0] = rho[2 * (t0 + q)];
-            st.rho[1] = rho[2 * (t0 + q) + 1];
-            tile[q] = st;
-        }
+        // cnt x kRootDoubles doubles, one per lane in turn
+        const double *src = reinterpret_cast<const double *>(roots + t0);
+        double *dst = reinterpret_cast<double *>(tile);
+        for (int q = threadIdx.x; q < cnt * kRootDoubles; q += blockDim.x) dst[q] = src[q];
         __syncthreads();
         for (int p = 0; p < cnt && !ovf; ++p) acc += tess_pair<kPot, kRadial>(P, tile[p], ratio, &ovf);
     }
@@ -200,6 +304,15 @@ __global__ void reduce_tess_kernel(const double *__restrict__ partial, int64_t c
     out[i] = scale * sum;
 }
 
+// Workspace: the chunk partials, then (256-byte aligned) one TessRoot per
+// tesseroid.
+inline size_t tess_partial_bytes(int64_t chunks, int64_t n_obs) {
+    return ((size_t)chunks * (size_t)n_obs * sizeof(double) + 255) / 256 * 256;
+}
+inline size_t tess_workspace_bytes(int64_t chunks, int64_t n_obs, int64_t n_tess) {
+    return tess_partial_bytes(chunks, n_obs) + (size_t)n_tess * sizeof(TessRoot);
+}
+
 __global__ void fill_zero_tess_kernel(double *__restrict__ out, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) out[i] = 0.0;
@@ -216,7 +329,8 @@ int hmx_tesseroid_workspace_bytes(int64_t n_obs, int64_t n_tesseroids, size_t *b
     if (n_obs < 0 || n_tesseroids < 0) return hmx::fail(HMX_ERR_ARG, "negative size");
     const int cu = hmx::device_cu_count();
     if (cu < 0) return cu;
-    *bytes = (size_t)hmx::forward_chunks(n_obs, n_tesseroids, cu) * (size_t)n_obs * sizeof(double);
+    *bytes = hmx::tess_workspace_bytes(hmx::forward_chunks(n_obs, n_tesseroids, cu), n_obs,
+                                       n_tesseroids);
     return HMX_OK;
 }
 
@@ -246,10 +360,15 @@ int hmx_tesseroid_gravity(const double *longitude, const double *latitude, const
     const int cu = device_cu_count();
     if (cu < 0) return cu;
     const int64_t chunks = forward_chunks(n_obs, n_tesseroids, cu);
-    const size_t need = (size_t)chunks * (size_t)n_obs * sizeof(double);
+    const size_t need = tess_workspace_bytes(chunks, n_obs, n_tesseroids);
     if (!workspace || workspace_bytes < need)
         return fail(HMX_ERR_ARG, "workspace too small: %zu < %zu bytes", workspace_bytes, need);
     double *partial = static_cast<double *>(workspace);
+    TessRoot *roots = reinterpret_cast<TessRoot *>(static_cast<char *>(workspace) +
+                                                   tess_partial_bytes(chunks, n_obs));
+    hipLaunchKernelGGL(tess_root_kernel, dim3((unsigned)((n_tesseroids + 255) / 256)), dim3(256),
+                       0, s, tesseroids, density_nodes, n_tesseroids, roots);
+    HMX_LAUNCHED("tess_root_kernel");
     const int64_t chunk_len = (n_tesseroids + chunks - 1) / chunks;
     const bool pot = field == HMX_POTENTIAL;
     // distance-size ratios of tesseroid_gravity.py:32
@@ -257,8 +376,8 @@ int hmx_tesseroid_gravity(const double *longitude, const double *latitude, const
     dim3 grid((unsigned)((n_obs + kObsPerBlock - 1) / kObsPerBlock), (unsigned)chunks);
 #define HMX_TESS(P, R)                                                                           \
     hipLaunchKernelGGL((tesseroid_kernel<P, R>), grid, dim3(kObsPerBlock), 0, s, longitude,     \
-                       latitude, radius, (int)n_obs, tesseroids, density_nodes, n_tesseroids,  \
-                       chunk_len, ratio, partial, overflow)
+                       latitude, radius, (int)n_obs, roots, n_tesseroids, chunk_len, ratio,     \
+                       partial, overflow)
     if (pot && radial_discretization) HMX_TESS(true, true);
     else if (pot) HMX_TESS(true, false);
     else if (radial_discretization) HMX_TESS(false, true);
