@@ -33,24 +33,25 @@ constexpr double kGlqNode = 0x1.279a74590331cp-1;
 // tess_root_kernel and staged in LDS: the box and the densities at the
 // radial nodes, the dimensions and centre of the discretisation test, and
 // the trigonometry, radii and masses of the 2 x 2 x 2 quadrature nodes.
-// Most pairs never divide the tesseroid, so these replace the ~18
-// trigonometric evaluations per pair of the generic path by products with
-// the point's own sines and cosines (cos(a - b) = cos a cos b + sin a sin b).
+// Most pairs never divide the tesseroid: for them these leave three cosines
+// of longitude differences per pair (the centre's and the two longitude
+// nodes'), taken as the reference does, cos(lambda - lambda_p): where the
+// point is close, 1 - cos(psi) is small and any other rounding of it would
+// show in the result.
 struct alignas(16) TessRoot {
     double t[6];    // w, e, s, n (degrees), bottom, top (m)
     double rho[2];  // density at the lower and upper radial node
     double l_lon, l_lat, l_rad;
-    double sin_lat_c, cos_lat_c, sin_lon_c, cos_lon_c, r_c;
-    double sin_lat_n[2], cos_lat_n[2], sin_lon_n[2], cos_lon_n[2], radius_n[2];
+    double sin_lat_c, cos_lat_c, lon_c, r_c;   // centre (lon_c in radians)
+    double sin_lat_n[2], cos_lat_n[2], lon_n[2], radius_n[2];  // nodes (lon_n in radians)
     double mass[2][2];  // [latitude node][radial node]: rho a r^2 cos(lat)
 };
 constexpr int kRootDoubles = sizeof(TessRoot) / sizeof(double);
-static_assert(sizeof(TessRoot) == 30 * sizeof(double), "TessRoot is 30 doubles");
+static_assert(sizeof(TessRoot) == 28 * sizeof(double), "TessRoot is 28 doubles");
 
 struct TessPoint {
     double lon, lat, r;            // degrees, degrees, m
     double lon_rad, cosphi, sinphi;
-    double coslon, sinlon;         // of lon_rad
 };
 
 // _tesseroid_dimensions (_tesseroid_utils.py): the longitudinal,
@@ -125,7 +126,7 @@ template <bool kPot>
 __device__ __forceinline__ double tess_glq_root(const TessPoint &P, const TessRoot &R) {
     double coslambda[2];
 #pragma unroll
-    for (int i = 0; i < 2; ++i) coslambda[i] = R.cos_lon_n[i] * P.coslon + R.sin_lon_n[i] * P.sinlon;
+    for (int i = 0; i < 2; ++i) coslambda[i] = cos(R.lon_n[i] - P.lon_rad);
     double result = 0.0;
 #pragma unroll
     for (int j = 0; j < 2; ++j)
@@ -180,7 +181,7 @@ template <bool kPot, bool kRadial>
 __device__ double tess_pair(const TessPoint &P, const TessRoot &R, double ratio, int *ovf) {
     {
         // _distance_tesseroid_point from the centre
-        const double coslambda = R.cos_lon_c * P.coslon + R.sin_lon_c * P.sinlon;
+        const double coslambda = cos(R.lon_c - P.lon_rad);
         const double cospsi = R.sin_lat_c * P.sinphi + R.cos_lat_c * P.cosphi * coslambda;
         const double dr = P.r - R.r_c;
         const double d = sqrt(dr * dr + 2.0 * P.r * R.r_c * (1.0 - cospsi));
@@ -239,14 +240,14 @@ __global__ __launch_bounds__(256) void tess_root_kernel(const double *__restrict
     tess_dimensions(R.t, R.l_lon, R.l_lat, R.l_rad);
     const double w = R.t[0], e = R.t[1], s = R.t[2], n_ = R.t[3], bottom = R.t[4], top = R.t[5];
     sincos((s + n_) / 2 * kDegT, &R.sin_lat_c, &R.cos_lat_c);
-    sincos((w + e) / 2 * kDegT, &R.sin_lon_c, &R.cos_lon_c);
+    R.lon_c = (w + e) / 2 * kDegT;
     R.r_c = (bottom + top) / 2;
     const double a_factor = 1.0 / 8.0 * ((e - w) * kDegT) * ((n_ - s) * kDegT) * (top - bottom);
     const double node[2] = {-kGlqNode, kGlqNode};
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
         sincos((0.5 * (n_ - s) * node[j] + 0.5 * (n_ + s)) * kDegT, &R.sin_lat_n[j], &R.cos_lat_n[j]);
-        sincos((0.5 * (e - w) * node[j] + 0.5 * (e + w)) * kDegT, &R.sin_lon_n[j], &R.cos_lon_n[j]);
+        R.lon_n[j] = (0.5 * (e - w) * node[j] + 0.5 * (e + w)) * kDegT;
         R.radius_n[j] = 0.5 * (top - bottom) * node[j] + 0.5 * (top + bottom);
     }
 #pragma unroll
@@ -273,7 +274,6 @@ __global__ __launch_bounds__(kObsPerBlock) void tesseroid_kernel(
     P.r = rad[ii];
     P.lon_rad = P.lon * kDegT;
     sincos(P.lat * kDegT, &P.sinphi, &P.cosphi);
-    sincos(P.lon_rad, &P.sinlon, &P.coslon);
     double acc = 0.0;
     int ovf = 0;
     const int64_t c0 = (int64_t)blockIdx.y * chunk_len;
