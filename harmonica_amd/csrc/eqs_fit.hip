@@ -552,62 +552,126 @@ __global__ __launch_bounds__(1024) void trsv_diag_kernel(const double *__restric
     }
 }
 
-// Forward substitution U^T y = r, update after the block at rows [p, p + nb)
-// (y_b already solved): r[c] -= sum_k U[p + k][c] y_b[k] for c in [p + nb, m),
-// 64 columns per workgroup, the 4 wavefronts taking 32 rows k each (64
-// consecutive columns per load, coalesced), partial sums added in order.
-__global__ __launch_bounds__(256) void trsv_fwd_update_kernel(const double *__restrict__ U,
-                                                              int64_t ld, int64_t m, int64_t p,
-                                                              int nb,
-                                                              const double *__restrict__ yb_g,
-                                                              double *__restrict__ r) {
+// Forward substitution U^T y = r, one block step: with y_b (rows
+// [p, p + kNB)) solved, r[c] -= sum_k U[p + k][c] y_b[k] for c in
+// [p + kNB, m), 128 columns per workgroup of 1024 threads (8 groups of 16
+// rows k, consecutive columns on consecutive lanes: coalesced; each lane's
+// 16 loads issued together; partial sums added in order).  Workgroup 0,
+// whose columns are the next block [p + kNB, p + kNB + nb2), then solves it
+// as trsv_diag_kernel does: y_next[t] = sum_{l <= t} Minv_next[l][t] r_next[l].
+__global__ __launch_bounds__(1024) void trsv_fwd_step_kernel(const double *__restrict__ U,
+                                                             int64_t ld, int64_t m, int64_t p,
+                                                             const double *__restrict__ Minv_next,
+                                                             int nb2, double *__restrict__ y,
+                                                             double *__restrict__ r) {
     __shared__ double yb[kNB];
-    __shared__ double part[4][64];
-    if (threadIdx.x < nb) yb[threadIdx.x] = yb_g[threadIdx.x];
+    __shared__ double part[8][kNB];
+    __shared__ double rnext[kNB];
+    const int t = threadIdx.x & (kNB - 1), g = threadIdx.x >> 7;
+    if (threadIdx.x < kNB) yb[threadIdx.x] = y[p + threadIdx.x];
     __syncthreads();
-    const int lane = threadIdx.x & 63, g = threadIdx.x >> 6;
-    const int64_t c = p + nb + (int64_t)blockIdx.x * 64 + lane;
+    const int64_t c = p + kNB + (int64_t)blockIdx.x * kNB + t;
     double v = 0.0;
     if (c < m) {
-        const int k1 = (g + 1) * 32 < nb ? (g + 1) * 32 : nb;
-#pragma unroll 8
-        for (int k = g * 32; k < k1; ++k) v = fma(U[(p + k) * ld + c], yb[k], v);
+        const double *col = U + (p + 16 * g) * ld + c;
+        double u[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) u[k] = col[k * ld];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) v = fma(u[k], yb[16 * g + k], v);
     }
-    part[g][lane] = v;
+    part[g][t] = v;
     __syncthreads();
-    if (g == 0 && c < m) r[c] -= ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane];
+    double rn = 0.0;
+    if (g == 0 && c < m) {
+        double sum = 0.0;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) sum += part[q][t];
+        rn = r[c] - sum;
+        r[c] = rn;
+    }
+    if (blockIdx.x != 0) return;  // uniform in the workgroup
+    if (g == 0) rnext[t] = rn;
+    __syncthreads();
+    double acc = 0.0;
+    if (t < nb2) {
+        double mv[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) mv[q] = Minv_next[(16 * g + q) * kNB + t];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int l = 16 * g + q;
+            acc = (l < nb2 && l <= t) ? fma(mv[q], rnext[l], acc) : acc;
+        }
+    }
+    part[g][t] = acc;
+    __syncthreads();
+    if (g == 0 && t < nb2) {
+        double sum = 0.0;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) sum += part[q][t];
+        y[p + kNB + t] = sum;
+    }
 }
 
-// Backward substitution U x = y, update after the block at rows [p, p + nb)
-// (x_b already solved): y[i] -= sum_c U[i][p + c] x_b[c] for i in [0, p).
-// Each wavefront takes 16 rows, 4 at a time; its 64 lanes read a row's nb
-// values coalesced and a butterfly sum combines them.
-__global__ __launch_bounds__(256) void trsv_bwd_update_kernel(const double *__restrict__ U,
-                                                              int64_t ld, int64_t p, int nb,
-                                                              const double *__restrict__ xb_g,
-                                                              double *__restrict__ y) {
+// Backward substitution U x = y, one block step: with x_b (rows [p, p + nb))
+// solved, y[i] -= sum_c U[i][p + c] x_b[c] for i in [0, p), p a multiple of
+// kNB.  Workgroup w (1024 threads) takes rows [p - kNB (w + 1), p - kNB w),
+// each of its 16 wavefronts 8 rows: the 64 lanes read a row's nb values
+// coalesced (the 8 rows' loads issued together) and a butterfly sums them.
+// Workgroup 0, whose rows are the next block [p - kNB, p), then solves it
+// as trsv_diag_kernel does: x_next[t] = sum_{l >= t} Minv_next[l][t] y_next[l].
+__global__ __launch_bounds__(1024) void trsv_bwd_step_kernel(const double *__restrict__ U,
+                                                             int64_t ld, int64_t p, int nb,
+                                                             const double *__restrict__ Minv_next,
+                                                             double *__restrict__ x,
+                                                             double *__restrict__ y) {
+    __shared__ double ynext[kNB];
+    __shared__ double part[8][kNB];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const double x0 = lane < nb ? xb_g[lane] : 0.0;
-    const double x1 = lane + 64 < nb ? xb_g[lane + 64] : 0.0;
-    for (int q = 0; q < 16; q += 4) {
-        const int64_t i0 = (int64_t)blockIdx.x * 64 + w * 16 + q;
-        double v[4];
+    const double x0 = lane < nb ? x[p + lane] : 0.0;
+    const double x1 = lane + 64 < nb ? x[p + lane + 64] : 0.0;
+    const int64_t rbase = p - (int64_t)kNB * (blockIdx.x + 1) + 8 * w;  // >= 0
+    double v[8];
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            const int64_t i = i0 + t;
-            const double *row = U + i * ld + p;
-            double a = (i < p && lane < nb) ? row[lane] * x0 : 0.0;
-            if (i < p && lane + 64 < nb) a = fma(row[lane + 64], x1, a);
-            v[t] = a;
-        }
+    for (int k = 0; k < 8; ++k) {
+        const double *row = U + (rbase + k) * ld + p;
+        double a = lane < nb ? row[lane] * x0 : 0.0;
+        if (lane + 64 < nb) a = fma(row[lane + 64], x1, a);
+        v[k] = a;
+    }
 #pragma unroll
-        for (int off = 32; off > 0; off >>= 1)
+    for (int off = 32; off > 0; off >>= 1)
 #pragma unroll
-            for (int t = 0; t < 4; ++t) v[t] += __shfl_xor(v[t], off, 64);
-        if (lane < 4) {
-            const double d = lane == 0 ? v[0] : lane == 1 ? v[1] : lane == 2 ? v[2] : v[3];
-            if (i0 + lane < p) y[i0 + lane] -= d;
-        }
+        for (int k = 0; k < 8; ++k) v[k] += __shfl_xor(v[k], off, 64);
+    if (lane < 8) {
+        double d = v[0];
+#pragma unroll
+        for (int k = 1; k < 8; ++k) d = lane == k ? v[k] : d;
+        const int64_t i = rbase + lane;
+        const double yn = y[i] - d;
+        y[i] = yn;
+        if (blockIdx.x == 0) ynext[8 * w + lane] = yn;
+    }
+    if (blockIdx.x != 0) return;  // uniform in the workgroup
+    __syncthreads();
+    const int t = threadIdx.x & (kNB - 1), g = threadIdx.x >> 7;
+    double mv[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) mv[q] = Minv_next[(16 * g + q) * kNB + t];
+    double acc = 0.0;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int l = 16 * g + q;
+        acc = l >= t ? fma(mv[q], ynext[l], acc) : acc;
+    }
+    part[g][t] = acc;
+    __syncthreads();
+    if (g == 0) {
+        double sum = 0.0;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) sum += part[q][t];
+        x[p - kNB + t] = sum;
     }
 }
 
@@ -1211,30 +1275,33 @@ struct Fit {
     // x = A^-1 r with the factor; r is overwritten (work), y is scratch.
     // Per kNB block: the diagonal-block product writes the block's solution,
     // then the update kernel applies it to the rest of the right-hand side.
+    // Blocked triangular solves: U^T y = r (forward), then U x = y
+    // (backward), with the diagonal blocks' inverses.  The first block of
+    // each direction is solved by trsv_diag_kernel; every later one by the
+    // update kernel of the block before it (its workgroup 0).  r and y are
+    // overwritten.
     int solve(double *r, double *y, double *x) {
-        for (int64_t p = 0; p < m; p += kNB) {
-            const int nb = (int)(m - p < kNB ? m - p : kNB);
-            const int64_t rest = m - p - nb;
-            hipLaunchKernelGGL(trsv_diag_kernel, dim3(1), dim3(1024), 0, s,
-                               Dinv + (p / kNB) * kNB * kNB, nb, true, r + p, y + p);
-            HMX_LAUNCHED("trsv_diag_kernel");
-            if (rest > 0) {
-                hipLaunchKernelGGL(trsv_fwd_update_kernel, dim3((unsigned)((rest + 63) / 64)),
-                                   dim3(256), 0, s, A, m, m, p, nb, y + p, r);
-                HMX_LAUNCHED("trsv_fwd_update_kernel");
-            }
+        const int nb0 = (int)(m < kNB ? m : kNB);
+        hipLaunchKernelGGL(trsv_diag_kernel, dim3(1), dim3(1024), 0, s, Dinv, nb0, true, r, y);
+        HMX_LAUNCHED("trsv_diag_kernel");
+        for (int64_t p = 0; p + kNB < m; p += kNB) {  // block p is full: another follows
+            const int64_t rest = m - p - kNB;
+            const int nb2 = (int)(rest < kNB ? rest : kNB);
+            hipLaunchKernelGGL(trsv_fwd_step_kernel, dim3((unsigned)((rest + kNB - 1) / kNB)),
+                               dim3(1024), 0, s, A, m, m, p, Dinv + (p / kNB + 1) * kNB * kNB,
+                               nb2, y, r);
+            HMX_LAUNCHED("trsv_fwd_step_kernel");
         }
         const int64_t last = ((m - 1) / kNB) * kNB;
-        for (int64_t p = last; p >= 0; p -= kNB) {
+        hipLaunchKernelGGL(trsv_diag_kernel, dim3(1), dim3(1024), 0, s,
+                           DinvT + (last / kNB) * kNB * kNB, (int)(m - last), false, y + last,
+                           x + last);
+        HMX_LAUNCHED("trsv_diag_kernel");
+        for (int64_t p = last; p > 0; p -= kNB) {
             const int nb = (int)(m - p < kNB ? m - p : kNB);
-            hipLaunchKernelGGL(trsv_diag_kernel, dim3(1), dim3(1024), 0, s,
-                               DinvT + (p / kNB) * kNB * kNB, nb, false, y + p, x + p);
-            HMX_LAUNCHED("trsv_diag_kernel");
-            if (p > 0) {
-                hipLaunchKernelGGL(trsv_bwd_update_kernel, dim3((unsigned)((p + 63) / 64)),
-                                   dim3(256), 0, s, A, m, p, nb, x + p, y);
-                HMX_LAUNCHED("trsv_bwd_update_kernel");
-            }
+            hipLaunchKernelGGL(trsv_bwd_step_kernel, dim3((unsigned)(p / kNB)), dim3(1024), 0, s,
+                               A, m, p, nb, DinvT + (p / kNB - 1) * kNB * kNB, x, y);
+            HMX_LAUNCHED("trsv_bwd_step_kernel");
         }
         return HMX_OK;
     }
