@@ -292,13 +292,14 @@ __device__ __forceinline__ void tile_k_loop(const Panel<T> &A, const Panel<T> &B
         __syncthreads();
     }
     // Epilogue (kept in this instantiation, so the accumulators never merge
-    // across the two loop versions).  With beta != 0 all old values of C are
-    // loaded first, from in-range (clamped) addresses without branches, so
-    // the loads overlap instead of completing one by one.
-    double old[4][4][4];
-    if (beta != 0.0) {
+    // across the two loop versions), one row of 16 x 16 tiles at a time: with
+    // beta != 0 the row's 16 old values of C are loaded first, from in-range
+    // (clamped) addresses without branches, so their loads overlap, while
+    // only a quarter of them is held at once.
 #pragma unroll
-        for (int ti = 0; ti < 4; ++ti)
+    for (int ti = 0; ti < 4; ++ti) {
+        double old[4][4];
+        if (beta != 0.0) {
 #pragma unroll
             for (int tj = 0; tj < 4; ++tj)
 #pragma unroll
@@ -307,11 +308,9 @@ __device__ __forceinline__ void tile_k_loop(const Panel<T> &A, const Panel<T> &B
                     int64_t c = c0 + 64 * wc + 16 * tj + (lane & 15);
                     r = r < M ? r : M - 1;
                     c = c < N ? c : N - 1;
-                    old[ti][tj][v] = C[r * ldc + c];
+                    old[tj][v] = C[r * ldc + c];
                 }
-    }
-#pragma unroll
-    for (int ti = 0; ti < 4; ++ti)
+        }
 #pragma unroll
         for (int tj = 0; tj < 4; ++tj)
 #pragma unroll
@@ -323,9 +322,10 @@ __device__ __forceinline__ void tile_k_loop(const Panel<T> &A, const Panel<T> &B
                                              : (double)acc[ti][tj][v];
                     double val = sign * sum;
                     if (r == c) val += shift;
-                    C[r * ldc + c] = beta == 0.0 ? val : fma(beta, old[ti][tj][v], val);
+                    C[r * ldc + c] = beta == 0.0 ? val : fma(beta, old[tj][v], val);
                 }
             }
+    }
 }
 
 // Upper-triangle tiles in an XCD-aware order, for the bf16 split-product
@@ -387,7 +387,7 @@ inline unsigned upper_tile_xcd_blocks(int64_t tiles) {
 // writes its tile, and tiles are disjoint, so C may alias B when M <= kST
 // (used by the in-place panel solve of the Cholesky factorisation).
 template <class T, bool UPPER>
-__global__ __launch_bounds__(256) void gemm_tn_kernel(Panel<T> A, Panel<T> B, int64_t K,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void gemm_tn_kernel(Panel<T> A, Panel<T> B, int64_t K,
                                                       int64_t M, int64_t N, double *C, int64_t ldc,
                                                       double beta, double sign, double shift,
                                                       int64_t tiles_n) {
