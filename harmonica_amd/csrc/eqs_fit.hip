@@ -459,10 +459,12 @@ __global__ __launch_bounds__(1024) void potrf_diag_kernel(double *__restrict__ D
     constexpr int S = kNB + 1;
     __shared__ double L[kNB * S];
     __shared__ double piv[kNB];
-    __shared__ double col[kNB];
     const int tid = threadIdx.x, tx = tid & 31, ty = tid >> 5;  // 32 x 32 threads
-    for (int r = ty; r < nb; r += 32)
-        for (int c = tx; c < nb; c += 32) L[r * S + c] = D[r * ld + c];
+    // upper triangle of the block; the rest of L is zero (the inverse below
+    // multiplies it by zeros, and it may not hold finite values in D)
+    for (int r = ty; r < kNB; r += 32)
+        for (int c = tx; c < kNB; c += 32)
+            L[r * S + c] = (r < nb && c < nb && r <= c) ? D[r * ld + c] : 0.0;
     __syncthreads();
     // Right-looking factorisation, one barrier per column: every thread
     // reads the pivot d (the same value, so the test is uniform) and updates
@@ -489,31 +491,45 @@ __global__ __launch_bounds__(1024) void potrf_diag_kernel(double *__restrict__ D
     __syncthreads();
     for (int r = ty; r < nb; r += 32)
         for (int c = tx; c < nb; c += 32) D[r * ld + c] = r <= c ? L[r * S + c] : 0.0;
-    // In-place inverse of the upper triangular factor, column by column:
-    // X[j][j] = 1 / U[j][j], X[i][j] = -X[j][j] sum_{k=i}^{j-1} X[i][k] U[k][j];
-    // each sum is split over the 8 adjacent lanes of its row and combined
-    // with shuffles.
-    const int ir = tid >> 3, part = tid & 7;
-    for (int j = 0; j < nb; ++j) {
-        if (tid < nb) col[tid] = tid <= j ? L[tid * S + j] : 0.0;
-        __syncthreads();
-        const double djj = 1.0 / col[j];
-        double acc = 0.0;
-        if (ir < j)
-            for (int k = ir + part; k < j; k += 8) acc = fma(L[ir * S + k], col[k], acc);
-        acc += __shfl_xor(acc, 1, 8);
-        acc += __shfl_xor(acc, 2, 8);
-        acc += __shfl_xor(acc, 4, 8);
-        if (part == 0 && ir < j) L[ir * S + j] = -djj * acc;
-        if (tid == 0) L[j * S + j] = djj;
-        __syncthreads();
-    }
-    for (int r = ty; r < kNB; r += 32)
-        for (int c = tx; c < kNB; c += 32) {
-            const double v = (r < nb && c < nb && r <= c) ? L[r * S + c] : 0.0;
-            Dinv[r * kNB + c] = v;
-            DinvT[c * kNB + r] = v;
+    // Inverse X = U^-1 row by row, from X U = I:
+    //   X[i][i] = 1 / U[i][i],  X[i][j] = -(sum_{k=i}^{j-1} X[i][k] U[k][j]) / U[j][j].
+    // Rows are independent, so there is no barrier: row i belongs to the 8
+    // adjacent lanes of its group, lane p keeping X[i][8 q + p] in xr[q]
+    // (zero until computed, and for k < i), each sum split over the lanes
+    // and combined with shuffles.  Wavefront w holds rows 8 w .. 8 w + 7,
+    // whose terms start at k = 8 w, so its loops skip q < w uniformly.
+    {
+        const int i = tid >> 3, p = tid & 7, w = tid >> 6;
+        const bool live = i < nb;
+        double xr[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) xr[q] = (live && 8 * q + p == i) ? 1.0 / L[i * S + i] : 0.0;
+#pragma unroll
+        for (int jb = 0; jb < 16; ++jb) {
+            if (jb < w) continue;  // uniform in the wavefront
+#pragma unroll
+            for (int jj = 0; jj < 8; ++jj) {
+                const int j = 8 * jb + jj;
+                double acc = 0.0;
+#pragma unroll
+                for (int q = 0; q <= jb; ++q) {
+                    if (q < w) continue;  // uniform in the wavefront
+                    acc = fma(xr[q], L[(8 * q + p) * S + j], acc);
+                }
+                acc += __shfl_xor(acc, 1, 8);
+                acc += __shfl_xor(acc, 2, 8);
+                acc += __shfl_xor(acc, 4, 8);
+                if (live && j > i && j < nb && p == jj) xr[jb] = -acc / L[j * S + j];
+            }
         }
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int k = 8 * q + p;
+            const double v = (live && k < nb && k >= i) ? xr[q] : 0.0;
+            Dinv[i * kNB + k] = v;
+            DinvT[k * kNB + i] = v;
+        }
+    }
 }
 
 // ------------------------------------------------------------------ solves
