@@ -521,8 +521,9 @@ def main():
         print(json.dumps({"copied_bytes": 1 << 30}))
         return
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path's "
-              "multi-GPU sharding is described in DESIGN.md", file=sys.stderr)
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): the "
+              "multi-GPU sharding of this path (observation tiles per GPU) is not built",
+              file=sys.stderr)
         sys.exit(2)
 
     if args.eqs_only:
