@@ -876,11 +876,11 @@ __global__ void zero_int_kernel(int *p) {
 // int8 exactly, products of two have at most 14 bits plus sign, and int32
 // MFMA sums are exact (one group's order sum is below 4 x 1024 x 2^14 =
 // 2^26).  The normal matrix is the bf16 formula with f_Gi f_Gj multiplied
-// by (2^32 / c)^2, a factor applied in FP64 when a group is folded.  Slice t, rows 16 g .. 16 g + 15 of column j
-// are the 16 consecutive int8 at ((t * n_units + g) * m + j) * 16 (byte q
-// of the unit is row 16 g + q), one 16-byte load per MFMA operand of
-// v_mfma_i32_16x16x64_i8, which takes twice the rows of the bf16 MFMA in
-// the same issue time.
+// by (2^32 / c)^2, a factor applied in FP64 when a group is folded.
+// Slice t, rows 16 g .. 16 g + 15 of column j are the 16 consecutive int8
+// at ((t * n_units + g) * m + j) * 16 (byte q of the unit is row
+// 16 g + q), one 16-byte load per MFMA operand of v_mfma_i32_16x16x64_i8,
+// which takes twice the rows of the bf16 MFMA in the same issue time.
 constexpr int kSlices = 4;
 constexpr int kFoldRows = 1024;  // rows per group (one scale per group and column)
 
@@ -1038,9 +1038,10 @@ __global__ __launch_bounds__(256) void slice_i8_kernel(
 // slice format Tr.  A workgroup of 8 wavefronts owns a kST x kST tile
 // (2 x 4 waves of 64 x 32, 4 x 2 MFMA 16x16 tiles each) and one exact
 // accumulator set per order (FP32 for bf16 slices, int32 for int8), folded
-// into FP64 at the end of every group of kFoldRows rows.  Slices 0..O1 of both panels go through two LDS
-// buffers, loaded straight from global memory into LDS, with the loads of
-// the next chunk in flight during the MFMAs.
+// into FP64 at the end of every group of kFoldRows rows.  Slices 0..O1 of
+// both panels go through two LDS buffers, loaded straight from global
+// memory into LDS, with the loads of the next chunk in flight during the
+// MFMAs.
 template <class Tr, int O0, int O1>
 __global__ __launch_bounds__(512) void gram_split_kernel(const typename Tr::unit_t *__restrict__ S,
                                                         int64_t n_groups, int64_t m,
