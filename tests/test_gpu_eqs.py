@@ -275,3 +275,23 @@ def test_split_normal_matrix_direct_solve(hm, precision, n_data, n_src):
     assert info["shift"] == 0.0, info
     err = np.abs(cmx - c64).max() / np.abs(c64).max()
     assert err <= 1e-5, err
+
+
+@pytest.mark.parametrize("precision", ["mixed", "mixed-bf16", "mixed-fp32"])
+@pytest.mark.parametrize("n_data,n_src", [(7, 5), (65, 33), (1025, 130)])
+def test_mixed_fit_small_and_ragged(hm, precision, n_data, n_src):
+    """Mixed fits where padding dominates: 7 data in one 64-row chunk (the
+    rest zero digits), 65 and 1025 data one row past a chunk and a row
+    group, 5 to 130 sources in partial 128-column tiles.  Refined to the
+    FP64 normal equations, the coefficients match the FP64 fit's."""
+    from harmonica_amd._eqs import fit_device
+
+    rng = np.random.default_rng(n_data)
+    L = 10e3
+    coords = (rng.uniform(0, L, n_data), rng.uniform(0, L, n_data), rng.uniform(500, 1500, n_data))
+    pts = (rng.uniform(0, L, n_src), rng.uniform(0, L, n_src), rng.uniform(-3000, -1000, n_src))
+    data = rng.normal(0, 1, n_data)
+    c64, _ = fit_device(coords, data, None, pts, 1.0, "float64")
+    cmx, info = fit_device(coords, data, None, pts, 1.0, precision)
+    assert np.all(np.isfinite(cmx)), info
+    npt.assert_allclose(cmx, c64, rtol=0, atol=1e-7 * np.abs(c64).max())
