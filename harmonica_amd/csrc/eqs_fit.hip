@@ -47,47 +47,77 @@ __device__ __forceinline__ double inv_dist(double ae, double an, double au, doub
 }
 
 // ------------------------------------------------------------ column stats
-// mean_j and population std s_j of column j of J, two passes in FP64.
-__global__ __launch_bounds__(256) void col_stats_kernel(
-    const double *__restrict__ e, const double *__restrict__ n, const double *__restrict__ u,
-    int64_t n_data, const double *__restrict__ pe, const double *__restrict__ pn,
-    const double *__restrict__ pu, int64_t n_src, double *__restrict__ scale) {
+// The reductions over data (column statistics, J^T products) and over
+// sources (J products) are split over blockIdx.y, so that outputs of a few
+// thousand elements (the windows of the gradient-boosted fit) still fill
+// the device: split y covers indices [y chunk, (y + 1) chunk) and writes
+// its partial sum to part[y * n_out + k]; a finish kernel adds the splits
+// in order.
+inline int64_t reduce_splits(int64_t n_out, int64_t n_red) {
+    const int64_t blocks = (n_out + 255) / 256;
+    int64_t splits = (1024 + blocks - 1) / blocks;  // about 1024 workgroups in all
+    const int64_t most = (n_red + 511) / 512;       // at least 512 terms per split
+    splits = splits < most ? splits : most;
+    return splits < 1 ? 1 : splits;
+}
+// Doubles of the partial sums for any reduce_splits(n_out, .) x n_out.
+inline size_t split_doubles(int64_t n_out) { return (size_t)(1024 + 1) * 256 + (size_t)n_out; }
+
+// Column moments of J over the split's data rows: pass 0 the sums of J_ij,
+// pass 1 the sums of (J_ij - mean_j)^2.
+__global__ __launch_bounds__(256) void col_moment_part_kernel(
+    int pass, const double *__restrict__ e, const double *__restrict__ n,
+    const double *__restrict__ u, int64_t n_data, const double *__restrict__ pe,
+    const double *__restrict__ pn, const double *__restrict__ pu, int64_t n_src,
+    const double *__restrict__ mean, int64_t chunk, double *__restrict__ part) {
     __shared__ double4_t tile[256];
     const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
     const int64_t jj = j < n_src ? j : n_src - 1;
     const double se = pe[jj], sn = pn[jj], su = pu[jj];
-    double sum = 0.0;
-    for (int pass = 0; pass < 2; ++pass) {
-        const double mean = sum / (double)n_data;
-        double acc = 0.0;
-        for (int64_t t0 = 0; t0 < n_data; t0 += 256) {
-            const int cnt = (int)(n_data - t0 < 256 ? n_data - t0 : 256);
-            __syncthreads();
-            if (threadIdx.x < cnt) {
-                double4_t v;
-                v.x = e[t0 + threadIdx.x];
-                v.y = n[t0 + threadIdx.x];
-                v.z = u[t0 + threadIdx.x];
-                v.w = 0.0;
-                tile[threadIdx.x] = v;
-            }
-            __syncthreads();
-            for (int q = 0; q < cnt; ++q) {
-                const double4_t o = tile[q];
-                const double g = inv_dist(o.x, o.y, o.z, se, sn, su);
-                if (pass == 0) {
-                    acc += g;
-                } else {
-                    const double dv = g - mean;
-                    acc = fma(dv, dv, acc);
-                }
+    const double mu = pass ? mean[jj] : 0.0;
+    const int64_t i0 = (int64_t)blockIdx.y * chunk;
+    const int64_t i1 = i0 + chunk < n_data ? i0 + chunk : n_data;
+    double acc = 0.0;
+    for (int64_t t0 = i0; t0 < i1; t0 += 256) {
+        const int cnt = (int)(i1 - t0 < 256 ? i1 - t0 : 256);
+        __syncthreads();
+        if (threadIdx.x < cnt) {
+            double4_t v;
+            v.x = e[t0 + threadIdx.x];
+            v.y = n[t0 + threadIdx.x];
+            v.z = u[t0 + threadIdx.x];
+            v.w = 0.0;
+            tile[threadIdx.x] = v;
+        }
+        __syncthreads();
+        for (int q = 0; q < cnt; ++q) {
+            const double4_t o = tile[q];
+            const double g = inv_dist(o.x, o.y, o.z, se, sn, su);
+            if (pass == 0) {
+                acc += g;
+            } else {
+                const double dv = g - mu;
+                acc = fma(dv, dv, acc);
             }
         }
-        sum = acc;
     }
-    if (j < n_src) {
-        double s = sqrt(sum / (double)n_data);
-        scale[j] = s == 0.0 ? 1.0 : s;
+    if (j < n_src) part[(int64_t)blockIdx.y * n_src + j] = acc;
+}
+
+// mean_j (pass 0) or the population std s_j (pass 1; 1 where it is 0) of
+// column j of J from its split sums.
+__global__ void col_moment_finish_kernel(int pass, const double *__restrict__ part,
+                                         int64_t splits, int64_t n_src, int64_t n_data,
+                                         double *__restrict__ out) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n_src) return;
+    double sum = 0.0;
+    for (int64_t y = 0; y < splits; ++y) sum += part[y * n_src + j];
+    if (pass == 0) {
+        out[j] = sum / (double)n_data;
+    } else {
+        const double sd = sqrt(sum / (double)n_data);
+        out[j] = sd == 0.0 ? 1.0 : sd;
     }
 }
 
@@ -692,19 +722,22 @@ __global__ __launch_bounds__(1024) void trsv_bwd_step_kernel(const double *__res
 }
 
 // ------------------------------------------------------ matrix-free A p
-// t_i = sum_j J_ij p_j / s_j (the unweighted data-space image of p).
+// t_i = sum_j J_ij p_j / s_j (the unweighted data-space image of p): the
+// split's sources [y chunk, (y + 1) chunk) into part[y * n_data + i].
 __global__ __launch_bounds__(256) void j_apply_kernel(
     const double *__restrict__ e, const double *__restrict__ n, const double *__restrict__ u,
     int64_t n_data, const double *__restrict__ pe, const double *__restrict__ pn,
     const double *__restrict__ pu, const double *__restrict__ p,
-    const double *__restrict__ scale, int64_t n_src, double *__restrict__ t) {
+    const double *__restrict__ scale, int64_t n_src, int64_t chunk, double *__restrict__ part) {
     __shared__ double4_t tile[256];
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     const int64_t ii = i < n_data ? i : n_data - 1;
     const double oe = e[ii], on = n[ii], ou = u[ii];
+    const int64_t j0 = (int64_t)blockIdx.y * chunk;
+    const int64_t j1 = j0 + chunk < n_src ? j0 + chunk : n_src;
     double acc = 0.0;
-    for (int64_t t0 = 0; t0 < n_src; t0 += 256) {
-        const int cnt = (int)(n_src - t0 < 256 ? n_src - t0 : 256);
+    for (int64_t t0 = j0; t0 < j1; t0 += 256) {
+        const int cnt = (int)(j1 - t0 < 256 ? j1 - t0 : 256);
         __syncthreads();
         if (threadIdx.x < cnt) {
             const int64_t j = t0 + threadIdx.x;
@@ -721,23 +754,35 @@ __global__ __launch_bounds__(256) void j_apply_kernel(
             acc = fma(s.w, inv_dist(oe, on, ou, s.x, s.y, s.z), acc);
         }
     }
-    if (i < n_data) t[i] = acc;
+    if (i < n_data) part[(int64_t)blockIdx.y * n_data + i] = acc;
 }
 
-// out_j = (1 / s_j) sum_i J_ij w_i t_i + alpha p_j  (w, p may be NULL).
+// t_i = sum over the splits, in order, of part[y * n_data + i].
+__global__ void j_apply_finish_kernel(const double *__restrict__ part, int64_t splits,
+                                      int64_t n_data, double *__restrict__ t) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_data) return;
+    double sum = 0.0;
+    for (int64_t y = 0; y < splits; ++y) sum += part[y * n_data + i];
+    t[i] = sum;
+}
+
+// sum_i J_ij w_i t_i (w may be NULL) over the split's data rows
+// [y chunk, (y + 1) chunk) into part[y * n_src + j].
 __global__ __launch_bounds__(256) void jt_apply_kernel(
     const double *__restrict__ e, const double *__restrict__ n, const double *__restrict__ u,
     const double *__restrict__ w, int64_t n_data, const double *__restrict__ pe,
     const double *__restrict__ pn, const double *__restrict__ pu, const double *__restrict__ t,
-    const double *__restrict__ scale, int64_t n_src, double alpha, const double *__restrict__ p,
-    double *__restrict__ out) {
+    int64_t n_src, int64_t chunk, double *__restrict__ part) {
     __shared__ double4_t tile[256];
     const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
     const int64_t jj = j < n_src ? j : n_src - 1;
     const double se = pe[jj], sn = pn[jj], su = pu[jj];
+    const int64_t i0 = (int64_t)blockIdx.y * chunk;
+    const int64_t i1 = i0 + chunk < n_data ? i0 + chunk : n_data;
     double acc = 0.0;
-    for (int64_t t0 = 0; t0 < n_data; t0 += 256) {
-        const int cnt = (int)(n_data - t0 < 256 ? n_data - t0 : 256);
+    for (int64_t t0 = i0; t0 < i1; t0 += 256) {
+        const int cnt = (int)(i1 - t0 < 256 ? i1 - t0 : 256);
         __syncthreads();
         if (threadIdx.x < cnt) {
             const int64_t i = t0 + threadIdx.x;
@@ -754,11 +799,22 @@ __global__ __launch_bounds__(256) void jt_apply_kernel(
             acc = fma(o.w, inv_dist(o.x, o.y, o.z, se, sn, su), acc);
         }
     }
-    if (j < n_src) {
-        double v = acc / scale[j];
-        if (p) v = fma(alpha, p[j], v);
-        out[j] = v;
-    }
+    if (j < n_src) part[(int64_t)blockIdx.y * n_src + j] = acc;
+}
+
+// out_j = (1 / s_j) (sum over the splits, in order, of part[y * n_src + j])
+// + alpha p_j  (p may be NULL).
+__global__ void jt_apply_finish_kernel(const double *__restrict__ part, int64_t splits,
+                                       const double *__restrict__ scale, int64_t n_src,
+                                       double alpha, const double *__restrict__ p,
+                                       double *__restrict__ out) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n_src) return;
+    double sum = 0.0;
+    for (int64_t y = 0; y < splits; ++y) sum += part[y * n_src + j];
+    double v = sum / scale[j];
+    if (p) v = fma(alpha, p[j], v);
+    out[j] = v;
 }
 
 // ------------------------------------------------------ small vector ops
@@ -1214,6 +1270,34 @@ struct Fit {
     double *scale, *A, *Dinv, *DinvT, *tmp, *part;
     double host_part[256];
     int *dinfo;
+    double *split = nullptr;  // partial sums of the split reductions
+
+    // out = (1 / s) J^T W t + alpha p (W = diag(w_), w_ and p may be NULL).
+    int jt(const double *w_, const double *t, double alpha_, const double *p, double *out) {
+        const int64_t sp = reduce_splits(m, nd), ch = (nd + sp - 1) / sp;
+        hipLaunchKernelGGL(jt_apply_kernel, dim3(blocks_for(m), (unsigned)sp), dim3(256), 0, s, e,
+                           n, u, w_, nd, pe, pn, pu, t, m, ch, split);
+        HMX_LAUNCHED("jt_apply_kernel");
+        hipLaunchKernelGGL(jt_apply_finish_kernel, dim3(blocks_for(m)), dim3(256), 0, s, split, sp,
+                           scale, m, alpha_, p, out);
+        HMX_LAUNCHED("jt_apply_finish_kernel");
+        return HMX_OK;
+    }
+
+    // scale_j = population std of column j of J (1 where it is 0), two passes.
+    int col_stats() {
+        const int64_t sp = reduce_splits(m, nd), ch = (nd + sp - 1) / sp;
+        for (int pass = 0; pass < 2; ++pass) {
+            hipLaunchKernelGGL(col_moment_part_kernel, dim3(blocks_for(m), (unsigned)sp),
+                               dim3(256), 0, s, pass, e, n, u, nd, pe, pn, pu, m,
+                               (const double *)scale, ch, split);
+            HMX_LAUNCHED("col_moment_part_kernel");
+            hipLaunchKernelGGL(col_moment_finish_kernel, dim3(blocks_for(m)), dim3(256), 0, s,
+                               pass, split, sp, m, nd, scale);
+            HMX_LAUNCHED("col_moment_finish_kernel");
+        }
+        return HMX_OK;
+    }
 
     int reduce(int mode, const double *x, const double *y, int64_t n, int64_t stride, double *out) {
         const int nb = 256;
@@ -1229,13 +1313,14 @@ struct Fit {
 
     // out = A p (FP64, matrix-free); t receives J S p (data space).
     int apply(const double *p, double *t, double *out) {
-        hipLaunchKernelGGL(j_apply_kernel, dim3(blocks_for(nd)), dim3(256), 0, s, e, n, u, nd, pe, pn,
-                           pu, p, scale, m, t);
+        const int64_t sp = reduce_splits(nd, m), ch = (m + sp - 1) / sp;
+        hipLaunchKernelGGL(j_apply_kernel, dim3(blocks_for(nd), (unsigned)sp), dim3(256), 0, s, e, n,
+                           u, nd, pe, pn, pu, p, scale, m, ch, split);
         HMX_LAUNCHED("j_apply_kernel");
-        hipLaunchKernelGGL(jt_apply_kernel, dim3(blocks_for(m)), dim3(256), 0, s, e, n, u, w, nd, pe,
-                           pn, pu, t, scale, m, alpha, p, out);
-        HMX_LAUNCHED("jt_apply_kernel");
-        return HMX_OK;
+        hipLaunchKernelGGL(j_apply_finish_kernel, dim3(blocks_for(nd)), dim3(256), 0, s, split, sp,
+                           nd, t);
+        HMX_LAUNCHED("j_apply_finish_kernel");
+        return jt(w, t, alpha, p, out);
     }
 
     // Blocked right-looking factorisation A = U^T U (upper, in place) in two
@@ -1449,7 +1534,8 @@ int hmx_eqs_fit_workspace_bytes(int64_t n_data, int64_t n_src, int precision_mod
     const size_t copies = precision_mode == HMX_FIT_FP64 ? 1 : 2;
     *bytes = copies * m * m * sizeof(double)             // A, then its factor U (+ a copy of A)
              + x_bytes + 2 * nblk * kNB * kNB * sizeof(double)  // inverses of diagonal blocks
-             + (10 * m + n + 256) * sizeof(double) + 256;  // 10 m-vectors, n-vector, partials, info
+             + (10 * m + n + 256) * sizeof(double)  // 10 m-vectors, n-vector, partials
+             + split_doubles(m > n ? m : n) * sizeof(double) + 256;  // split sums, info
     return HMX_OK;
 }
 
@@ -1522,17 +1608,15 @@ int hmx_eqs_fit(const double *easting, const double *northing, const double *upw
            *wk2 = wk + m, *pow2 = wk2 + m;
     f.tmp = pow2 + m;  // n_data
     f.part = f.tmp + nd;
-    f.dinfo = reinterpret_cast<int *>(f.part + 256);
+    f.split = f.part + 256;
+    f.dinfo = reinterpret_cast<int *>(f.split + split_doubles(m > nd ? m : nd));
     const unsigned gm = blocks_for(m), gy = (unsigned)(nd < 8192 ? nd : 8192);
 
     // 1. column scales, right-hand side b = S J^T W d (FP64).
-    hipLaunchKernelGGL(col_stats_kernel, dim3(gm), dim3(256), 0, s, easting, northing, upward, nd,
-                       p_easting, p_northing, p_upward, m, f.scale);
-    HMX_LAUNCHED("col_stats_kernel");
-    hipLaunchKernelGGL(jt_apply_kernel, dim3(gm), dim3(256), 0, s, easting, northing, upward,
-                       weights, nd, p_easting, p_northing, p_upward, data, f.scale, m, 0.0,
-                       (const double *)nullptr, b);
-    HMX_LAUNCHED("jt_apply_kernel");
+    st = f.col_stats();
+    if (st) return st;
+    st = f.jt(weights, data, 0.0, nullptr, b);
+    if (st) return st;
 
     // 2. X = W^1/2 J S (or its slices) and the normal matrix in the mode's
     // precision.
