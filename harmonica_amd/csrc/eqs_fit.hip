@@ -521,8 +521,11 @@ __global__ __launch_bounds__(1024) void potrf_diag_kernel(double *__restrict__ D
     __syncthreads();
     for (int r = ty; r < nb; r += 32)
         for (int c = tx; c < nb; c += 32) D[r * ld + c] = r <= c ? L[r * S + c] : 0.0;
+    // reciprocals of the factor's diagonal, once (they multiply below)
+    if (tid < nb) piv[tid] = 1.0 / L[tid * S + tid];
+    __syncthreads();
     // Inverse X = U^-1 row by row, from X U = I:
-    //   X[i][i] = 1 / U[i][i],  X[i][j] = -(sum_{k=i}^{j-1} X[i][k] U[k][j]) / U[j][j].
+    //   X[i][i] = 1 / U[i][i],  X[i][j] = -(sum_{k=i}^{j-1} X[i][k] U[k][j]) (1 / U[j][j]).
     // Rows are independent, so there is no barrier: row i belongs to the 8
     // adjacent lanes of its group, lane p keeping X[i][8 q + p] in xr[q]
     // (zero until computed, and for k < i), each sum split over the lanes
@@ -533,23 +536,27 @@ __global__ __launch_bounds__(1024) void potrf_diag_kernel(double *__restrict__ D
         const bool live = i < nb;
         double xr[16];
 #pragma unroll
-        for (int q = 0; q < 16; ++q) xr[q] = (live && 8 * q + p == i) ? 1.0 / L[i * S + i] : 0.0;
+        for (int q = 0; q < 16; ++q) xr[q] = (live && 8 * q + p == i) ? piv[i] : 0.0;
 #pragma unroll
         for (int jb = 0; jb < 16; ++jb) {
             if (jb < w) continue;  // uniform in the wavefront
 #pragma unroll
             for (int jj = 0; jj < 8; ++jj) {
                 const int j = 8 * jb + jj;
-                double acc = 0.0;
+                double acc0 = 0.0, acc1 = 0.0;  // two chains of FMAs
 #pragma unroll
                 for (int q = 0; q <= jb; ++q) {
                     if (q < w) continue;  // uniform in the wavefront
-                    acc = fma(xr[q], L[(8 * q + p) * S + j], acc);
+                    if (q & 1)
+                        acc1 = fma(xr[q], L[(8 * q + p) * S + j], acc1);
+                    else
+                        acc0 = fma(xr[q], L[(8 * q + p) * S + j], acc0);
                 }
+                double acc = acc0 + acc1;
                 acc += __shfl_xor(acc, 1, 8);
                 acc += __shfl_xor(acc, 2, 8);
                 acc += __shfl_xor(acc, 4, 8);
-                if (live && j > i && j < nb && p == jj) xr[jb] = -acc / L[j * S + j];
+                if (live && j > i && j < nb && p == jj) xr[jb] = -acc * piv[j];
             }
         }
 #pragma unroll
