@@ -476,6 +476,28 @@ int launch_gemm_tn(const double *A, int64_t lda, const double *B, int64_t ldb, i
 }
 
 // ---------------------------------------------------------------- Cholesky
+// v from another lane of the same group of 8 aligned lanes, by a DPP move
+// of its two 32-bit halves (kCtrl: quad_perm [1,0,3,2] = 0xb1 for lane ^ 1,
+// quad_perm [2,3,0,1] = 0x4e for lane ^ 2, row_half_mirror = 0x141 for
+// lane 7 - i of the group); a VALU operation, no trip through the LDS unit.
+template <int kCtrl>
+__device__ __forceinline__ double dpp_f64(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)b, kCtrl, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), kCtrl, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// Sum over each group of 8 aligned lanes, in every lane of the group: the
+// pair and quad sums, then the other quad's sum (the same value in its four
+// lanes), so the result is the same as with xor shuffles 1, 2, 4.
+__device__ __forceinline__ double sum8_f64(double v) {
+    v += dpp_f64<0xb1>(v);
+    v += dpp_f64<0x4e>(v);
+    v += dpp_f64<0x141>(v);
+    return v;
+}
+
 // Factor the nb x nb diagonal block at D (upper, row-major, ld) in place as
 // U^T U and write U^-1 to Dinv (nb x nb, row-major, ld kNB) and its
 // transpose to DinvT.  One workgroup; the block lives in LDS.  On a
@@ -529,7 +551,7 @@ __global__ __launch_bounds__(1024) void potrf_diag_kernel(double *__restrict__ D
     // Rows are independent, so there is no barrier: row i belongs to the 8
     // adjacent lanes of its group, lane p keeping X[i][8 q + p] in xr[q]
     // (zero until computed, and for k < i), each sum split over the lanes
-    // and combined with shuffles.  Wavefront w holds rows 8 w .. 8 w + 7,
+    // and combined by DPP moves.  Wavefront w holds rows 8 w .. 8 w + 7,
     // whose terms start at k = 8 w, so its loops skip q < w uniformly.
     {
         const int i = tid >> 3, p = tid & 7, w = tid >> 6;
@@ -552,10 +574,7 @@ __global__ __launch_bounds__(1024) void potrf_diag_kernel(double *__restrict__ D
                     else
                         acc0 = fma(xr[q], L[(8 * q + p) * S + j], acc0);
                 }
-                double acc = acc0 + acc1;
-                acc += __shfl_xor(acc, 1, 8);
-                acc += __shfl_xor(acc, 2, 8);
-                acc += __shfl_xor(acc, 4, 8);
+                const double acc = sum8_f64(acc0 + acc1);
                 if (live && j > i && j < nb && p == jj) xr[jb] = -acc * piv[j];
             }
         }
