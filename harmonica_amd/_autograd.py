@@ -76,7 +76,12 @@ class LayerGeometry:
             self.easting.numel(), ptr(self.northing), self.northing.numel(), ptr(self.bottom),
             ptr(top.contiguous()), ptr(density.contiguous()), self.threshold, ptr(gs), ptr(gf),
             ptr(self._ws), self._ws_bytes, stream_handle(None)))
-        return gs * _GZ_FACTOR, gf * density * _GZ_FACTOR
+        # The kernel returns zero sums for the cells the layer loop skips
+        # (NaN density, NaN top or bottom, thin cells); the density weight of
+        # the top derivative is applied only where it is finite, so a NaN
+        # density gives a zero derivative instead of 0 * NaN.
+        weight = torch.where(torch.isfinite(density), density, torch.zeros_like(density))
+        return gs * _GZ_FACTOR, gf * weight * _GZ_FACTOR
 
 
 class _LayerGz(torch.autograd.Function):
@@ -114,19 +119,25 @@ def gauss_newton(geometry, data, density0, top0, n_iterations=20, damping=1e-2, 
     top = top0.clone()
     rho_ref = rho.clone() if density_ref is None else density_ref
     top_ref = top.clone() if top_ref is None else top_ref
+    # Cells with a NaN density or top are holes the forward model skips:
+    # they stay fixed (their update is zero), and their regularisation
+    # terms are left out so that NaN does not reach the step.
+    free = torch.isfinite(rho) & torch.isfinite(top)
+    zero = torch.zeros_like(rho)
     history = []
     for _ in range(n_iterations):
         r = geometry.forward(rho, top) - data
         history.append(float(torch.sqrt(torch.mean(r * r))))
         # gradient of the objective in scaled variables (x = rho/s_rho, y = T/s_T)
         g_rho, g_top = geometry.vjp(rho, top, r)
-        b_rho = -(g_rho * density_scale + damping * (rho - rho_ref) / density_scale)
-        b_top = -(g_top * top_scale + damping * (top - top_ref) / top_scale)
+        b_rho = torch.where(free, -(g_rho * density_scale + damping * (rho - rho_ref) / density_scale), zero)
+        b_top = torch.where(free, -(g_top * top_scale + damping * (top - top_ref) / top_scale), zero)
 
         def normal(p_rho, p_top):
             jv = geometry.jvp(rho, top, p_rho * density_scale, p_top * top_scale)
             a_rho, a_top = geometry.vjp(rho, top, jv)
-            return a_rho * density_scale + damping * p_rho, a_top * top_scale + damping * p_top
+            return (torch.where(free, a_rho * density_scale + damping * p_rho, zero),
+                    torch.where(free, a_top * top_scale + damping * p_top, zero))
 
         x_rho = torch.zeros_like(rho)
         x_top = torch.zeros_like(top)
@@ -147,8 +158,8 @@ def gauss_newton(geometry, data, density0, top0, n_iterations=20, damping=1e-2, 
             p_rho = res_rho + (rr_new / rr) * p_rho
             p_top = res_top + (rr_new / rr) * p_top
             rr = rr_new
-        rho = rho + x_rho * density_scale
-        top = top + x_top * top_scale
+        rho = torch.where(free, rho + x_rho * density_scale, rho)
+        top = torch.where(free, top + x_top * top_scale, top)
     r = geometry.forward(rho, top) - data
     history.append(float(torch.sqrt(torch.mean(r * r))))
     return rho, top, history

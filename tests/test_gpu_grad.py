@@ -109,3 +109,50 @@ def test_gauss_newton_reduces_misfit(setup):
     top0 = dev(np.full_like(top, -800.0))
     rho, tp, history = ag.gauss_newton(geom, data, rho0, top0, n_iterations=8, damping=1e-3)
     assert history[-1] < 0.05 * history[0], history
+
+
+def test_gradients_with_nan_holes(setup):
+    """Cells the layer loop skips (NaN density, NaN top) have zero
+    derivatives, the others keep theirs (central differences), and a
+    Gauss-Newton inversion over the holed layer stays finite and leaves the
+    holes as they are."""
+    import harmonica_amd as hm
+
+    torch, ag, geom, dev, easting, northing, top, bottom, density, coords, rng = setup
+    density = density.copy()
+    top = top.copy()
+    density[2, 3] = np.nan
+    density[6, 10] = np.nan
+    top[4, 4] = np.nan
+    rho_t = dev(density).requires_grad_(True)
+    top_t = dev(top).requires_grad_(True)
+    v = rng.normal(size=coords[0].size)
+    (ag.layer_gz(rho_t, top_t, geom) * dev(v)).sum().backward()
+    g_rho, g_top = rho_t.grad.cpu().numpy(), top_t.grad.cpu().numpy()
+    assert np.isfinite(g_rho).all() and np.isfinite(g_top).all()
+    for k, j in [(2, 3), (6, 10), (4, 4)]:
+        assert g_rho[k, j] == 0.0 and g_top[k, j] == 0.0
+    pts = [c.ravel() for c in coords]
+    k, j = 3, 3
+    p = np.array(_prism(easting, northing, bottom, top, k, j))
+    hr, ht = 1e-4, 1e-3
+    fr = (v @ hm.prism_gravity(pts, p, density[k, j] + hr, "g_z")
+          - v @ hm.prism_gravity(pts, p, density[k, j] - hr, "g_z")) / (2 * hr)
+    pu, pd = p.copy(), p.copy()
+    pu[5] += ht
+    pd[5] -= ht
+    ft = (v @ hm.prism_gravity(pts, pu, density[k, j], "g_z")
+          - v @ hm.prism_gravity(pts, pd, density[k, j], "g_z")) / (2 * ht)
+    npt.assert_allclose(g_rho[k, j], fr, rtol=1e-8)
+    npt.assert_allclose(g_top[k, j], ft, rtol=1e-6)
+
+    data = geom.forward(dev(density), dev(top))
+    rho0 = np.where(np.isnan(density), np.nan, -300.0)
+    top0 = np.where(np.isnan(top), np.nan, -800.0)
+    rho, tp, history = ag.gauss_newton(geom, data, dev(rho0), dev(top0), n_iterations=6,
+                                       damping=1e-3)
+    assert np.isfinite(history).all() and history[-1] < 0.1 * history[0], history
+    rho, tp = rho.cpu().numpy(), tp.cpu().numpy()
+    assert np.isnan(rho[2, 3]) and np.isnan(rho[6, 10]) and np.isnan(tp[4, 4])
+    assert np.isfinite(rho[~np.isnan(density)]).all()
+    assert np.isfinite(tp[~np.isnan(top)]).all()
