@@ -251,16 +251,26 @@ def fit_device(coords, data, weights, points, damping, precision="float64", max_
         w = None if w is None else w[idx].contiguous()
     n, m = c[0].numel(), p[0].numel()
     nbytes = ctypes.c_size_t(0)
+    if damping is None:  # LinearRegression's minimum-norm least squares
+        mode |= _lib.HMX_FIT_NO_DAMPING
     check(lib().hmx_eqs_fit_workspace_bytes(n, m, mode, ctypes.byref(nbytes)))
     ws = torch.empty(nbytes.value, dtype=torch.uint8, device=c[0].device)
     coefs = torch.empty(m, dtype=torch.float64, device=c[0].device)
-    info = (ctypes.c_double * 5)()
+    info = (ctypes.c_double * 8)()
     check(lib().hmx_eqs_fit(*(ptr(x) for x in c), n, ptr(d), ptr(w), *(ptr(x) for x in p), m,
                             float(0.0 if damping is None else damping), mode, int(max_iter),
                             float(rtol), ptr(coefs), info, ptr(ws), nbytes.value,
                             stream_handle(None)))
-    keys = ("iterations", "last_relative_update", "shift", "retries", "max_diagonal")
-    return coefs.cpu().numpy(), dict(zip(keys, list(info)))
+    return coefs.cpu().numpy(), dict(zip(FIT_INFO_KEYS, list(info)))
+
+
+#: Keys of the solver statistics of hmx_eqs_fit (``fit_info_``): conjugate
+#: gradient iterations, their last relative update, the diagonal shift and
+#: factorisation retries (fits larger than the SVD route), the largest
+#: diagonal entry of the normal matrix, whether the SVD route solved the fit
+#: (1.0) and its Jacobi sweeps and kept singular values.
+FIT_INFO_KEYS = ("iterations", "last_relative_update", "shift", "retries", "max_diagonal",
+                 "svd", "svd_sweeps", "svd_rank")
 
 
 # ----------------------------------------------------------- the estimator

@@ -28,6 +28,7 @@ import torch
 
 from . import _lib
 from ._eqs import (
+    FIT_INFO_KEYS,
     EquivalentSources,
     cast_fit_input,
     check_fit_input,
@@ -171,13 +172,15 @@ class EquivalentSourcesGB(EquivalentSources):
         mode = _MODES[self.fit_precision]
         max_iter = 10 if mode == _lib.HMX_FIT_FP64 else 300
         rtol = 1e-12 if mode == _lib.HMX_FIT_FP64 else 1e-9
+        if self.damping is None:  # LinearRegression's minimum-norm least squares
+            mode |= _lib.HMX_FIT_NO_DAMPING
         nbytes = ctypes.c_size_t(0)
         check(lib().hmx_eqs_gb_fit_workspace_bytes(n, max_wd, max_ws, mode, ctypes.byref(nbytes)))
         ws = torch.empty(nbytes.value, dtype=torch.uint8, device=dev)
         coefs = torch.empty(m, dtype=torch.float64, device=dev)
         residue = torch.empty(n, dtype=torch.float64, device=dev)
         rmse = torch.empty(n_windows + 1, dtype=torch.float64, device=dev)
-        info = (ctypes.c_double * 5)()
+        info = (ctypes.c_double * 8)()
         i64p = ctypes.POINTER(ctypes.c_int64)
         check(lib().hmx_eqs_gb_fit(
             *(ptr(x) for x in c), n, ptr(d), ptr(w), *(ptr(x) for x in p), m, ptr(didx),
@@ -187,8 +190,9 @@ class EquivalentSourcesGB(EquivalentSources):
             stream_handle(None)))
         self.coefs_ = coefs.cpu().numpy().astype(self.points_[0].dtype, copy=False)
         self.rmse_per_iteration_ = rmse.cpu().numpy()
-        keys = ("iterations", "last_relative_update", "shift", "retries", "max_diagonal")
-        self.fit_info_ = dict(zip(keys, list(info)))
+        # summed over the windows, except the last update and the largest
+        # shift and diagonal; "svd": windows solved by the SVD route
+        self.fit_info_ = dict(zip(FIT_INFO_KEYS, list(info)))
         self.fit_info_["windows"] = n_windows
 
     def _create_windows(self, coordinates, shuffle=True):

@@ -30,6 +30,10 @@ HMX_FIT_FP64 = 0
 HMX_FIT_FP32_MFMA = 1
 HMX_FIT_BF16_SPLIT = 2
 HMX_FIT_INT8_SPLIT = 3
+HMX_FIT_NO_DAMPING = 0x100
+HMX_SVD_RIDGE = 0
+HMX_SVD_LSTSQ = 1
+HMX_SVD_DUAL_LSTSQ = 2
 
 #: Field names in the reference's FIELDS order (gravity.py:37-48); bit i of a
 #: field mask selects FIELDS[i].
@@ -62,6 +66,9 @@ SIGNATURES = {
     "hmx_eqs_fit_workspace_bytes": (_int, [_i64, _i64, _int, _szp]),
     "hmx_eqs_fit": (_int, [_vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _i64, _dbl, _int, _int,
                            _dbl, _vp, ctypes.POINTER(ctypes.c_double), _vp, ctypes.c_size_t, _vp]),
+    "hmx_lsq_svd_workspace_bytes": (_int, [_i64, _i64, _szp]),
+    "hmx_lsq_svd": (_int, [_vp, _i64, _i64, _vp, _dbl, _int, _dbl, _vp, _vp,
+                           ctypes.POINTER(ctypes.c_double), _vp, ctypes.c_size_t, _vp]),
     "hmx_tesseroid_workspace_bytes": (_int, [_i64, _i64, _szp]),
     "hmx_tesseroid_gravity": (_int, [_vp, _vp, _vp, _i64, _vp, _vp, _i64, _u32, _int, _vp, _vp,
                                      _vp, ctypes.c_size_t, _vp]),

@@ -138,6 +138,27 @@ __global__ __launch_bounds__(256) void scaled_jacobian_kernel(
     }
 }
 
+// G[j * n_data + i] = sqrt(w_i) * J_ij / s_j: the scaled jacobian
+// column-major (columns contiguous), for the SVD route.
+__global__ __launch_bounds__(256) void scaled_jacobian_cm_kernel(
+    const double *__restrict__ e, const double *__restrict__ n, const double *__restrict__ u,
+    const double *__restrict__ w, int64_t n_data, const double *__restrict__ pe,
+    const double *__restrict__ pn, const double *__restrict__ pu,
+    const double *__restrict__ scale, int64_t n_src, double *__restrict__ G) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n_data) return;
+    const double oe = e[i], on = n[i], ou = u[i], sw = w ? sqrt(w[i]) : 1.0;
+    for (int64_t j = blockIdx.y; j < n_src; j += gridDim.y)
+        G[j * n_data + i] = sw * inv_dist(oe, on, ou, pe[j], pn[j], pu[j]) * (1.0 / scale[j]);
+}
+
+// y_i = sqrt(w_i) d_i (w == NULL -> d).
+__global__ void sqrtw_times_kernel(int64_t n, const double *__restrict__ w,
+                                   const double *__restrict__ d, double *__restrict__ y) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = w ? sqrt(w[i]) * d[i] : d[i];
+}
+
 // -------------------------------------------------------------- MFMA SYRK
 template <class T>
 struct Mfma;
@@ -874,24 +895,25 @@ __global__ void copy_kernel(int64_t n, const double *__restrict__ x, double *__r
     if (i < n) y[i] = x[i];
 }
 
-// Partial dot products / max-abs (mode 0 / 1): one value per workgroup,
-// combined on the host in workgroup order (deterministic).
+// Partial dot products / max-abs / min-abs (mode 0 / 1 / 2): one value per
+// workgroup, combined on the host in workgroup order (deterministic).
 __global__ __launch_bounds__(256) void reduce_partial_kernel(int mode, int64_t n, int64_t stride,
                                                              const double *__restrict__ x,
                                                              const double *__restrict__ y,
                                                              double *__restrict__ part) {
     __shared__ double red[256];
-    double acc = 0.0;
+    double acc = mode == 2 ? INFINITY : 0.0;
     for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
         const double v = x[i * stride];
-        acc = mode == 0 ? fma(v, y[i], acc) : fmax(acc, fabs(v));
+        acc = mode == 0 ? fma(v, y[i], acc) : mode == 1 ? fmax(acc, fabs(v)) : fmin(acc, fabs(v));
     }
     red[threadIdx.x] = acc;
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
         if (threadIdx.x < s)
-            red[threadIdx.x] = mode == 0 ? red[threadIdx.x] + red[threadIdx.x + s]
-                                         : fmax(red[threadIdx.x], red[threadIdx.x + s]);
+            red[threadIdx.x] = mode == 0   ? red[threadIdx.x] + red[threadIdx.x + s]
+                               : mode == 1 ? fmax(red[threadIdx.x], red[threadIdx.x + s])
+                                           : fmin(red[threadIdx.x], red[threadIdx.x + s]);
         __syncthreads();
     }
     if (threadIdx.x == 0) part[blockIdx.x] = red[0];
@@ -1286,6 +1308,43 @@ inline unsigned blocks_for(int64_t n, int per = 256) {
     return (unsigned)(b < 1 ? 1 : b);
 }
 
+// Whether a fit takes the SVD route when its Cholesky route cannot match
+// the reference's solver (the route is sized for these fits only).
+inline bool svd_route_fits(int64_t n_data, int64_t n_src) {
+    return n_src <= kSvdMaxSrc && n_data * n_src <= kSvdMaxElems;
+}
+
+// Bytes of the scaled-jacobian region of the fit's workspace: the mode's
+// jacobian (FP64, FP32, or slices with their group scales), enlarged where
+// needed so that, with the copy of the normal matrix in front of it (mixed
+// modes), it holds the FP64 column-major jacobian of the SVD route.
+inline size_t fit_x_bytes(int64_t n_data, int64_t n_src, int mode) {
+    const size_t m = (size_t)n_src, n = (size_t)n_data;
+    size_t x_bytes = n * m * sizeof(double);  // X = W^1/2 J S
+    if (mode == HMX_FIT_FP32_MFMA) x_bytes = n * m * sizeof(float);
+    if (mode == HMX_FIT_BF16_SPLIT || mode == HMX_FIT_INT8_SPLIT) {
+        // slices (16-byte units of 8 or 16 rows, padded to whole chunks of 4
+        // units) and the group scales
+        const size_t rows = mode == HMX_FIT_INT8_SPLIT ? 16 : 8;
+        const size_t n_units = (n + 4 * rows - 1) / (4 * rows) * 4;
+        x_bytes = kSlices * n_units * m * 16 +
+                  (n_units * rows + kFoldRows - 1) / kFoldRows * m * sizeof(double);
+    }
+    if (svd_route_fits(n_data, n_src)) {
+        const size_t avail = (mode == HMX_FIT_FP64 ? 0 : m * m * sizeof(double)) + x_bytes;
+        const size_t g_bytes = n * m * sizeof(double);
+        if (avail < g_bytes) x_bytes += g_bytes - avail;
+    }
+    return x_bytes;
+}
+
+// Pivot ratio below which an unregularised fit (damping=None) is solved by
+// the SVD route: min U_ii^2 / max U_ii^2 >= 1 / cond(X)^2, so above it the
+// scaled jacobian X has cond(X) < 1e6 and lstsq's cut-off (max(n, m)
+// 2^-52 s_max) drops nothing: the least-squares solution is unique and the
+// refined Cholesky solution is it.
+constexpr double kLstsqPivotRatio = 1e-12;
+
 struct Fit {
     const double *e, *n, *u, *w;
     int64_t nd;
@@ -1331,8 +1390,11 @@ struct Fit {
         HMX_LAUNCHED("reduce_partial_kernel");
         HMX_HIP(hipMemcpyAsync(host_part, part, nb * sizeof(double), hipMemcpyDeviceToHost, s));
         HMX_HIP(hipStreamSynchronize(s));
-        double acc = 0.0;
-        for (int i = 0; i < nb; ++i) acc = mode == 0 ? acc + host_part[i] : std::fmax(acc, host_part[i]);
+        double acc = mode == 2 ? INFINITY : 0.0;
+        for (int i = 0; i < nb; ++i)
+            acc = mode == 0   ? acc + host_part[i]
+                  : mode == 1 ? std::fmax(acc, host_part[i])
+                              : std::fmin(acc, host_part[i]);
         *out = acc;
         return HMX_OK;
     }
@@ -1542,24 +1604,16 @@ int hmx_eqs_fit_workspace_bytes(int64_t n_data, int64_t n_src, int precision_mod
     using namespace hmx;
     if (!bytes) return fail(HMX_ERR_ARG, "bytes is NULL");
     if (n_data <= 0 || n_src <= 0) return fail(HMX_ERR_ARG, "empty problem");
-    if (precision_mode != HMX_FIT_FP64 && precision_mode != HMX_FIT_FP32_MFMA &&
-        precision_mode != HMX_FIT_BF16_SPLIT && precision_mode != HMX_FIT_INT8_SPLIT)
+    const int mode = precision_mode & ~HMX_FIT_NO_DAMPING;
+    if (mode != HMX_FIT_FP64 && mode != HMX_FIT_FP32_MFMA && mode != HMX_FIT_BF16_SPLIT &&
+        mode != HMX_FIT_INT8_SPLIT)
         return fail(HMX_ERR_ARG, "invalid precision mode %d", precision_mode);
     const size_t m = (size_t)n_src, n = (size_t)n_data;
     const size_t nblk = (m + kNB - 1) / kNB;
-    size_t x_bytes = n * m * sizeof(double);                        // X = W^1/2 J S
-    if (precision_mode == HMX_FIT_FP32_MFMA) x_bytes = n * m * sizeof(float);
-    if (precision_mode == HMX_FIT_BF16_SPLIT || precision_mode == HMX_FIT_INT8_SPLIT) {
-        // slices (16-byte units of 8 or 16 rows, padded to whole chunks of 4
-        // units) and the group scales
-        const size_t rows = precision_mode == HMX_FIT_INT8_SPLIT ? 16 : 8;
-        const size_t n_units = (n + 4 * rows - 1) / (4 * rows) * 4;
-        x_bytes = kSlices * n_units * m * 16 +
-                  (n_units * rows + kFoldRows - 1) / kFoldRows * m * sizeof(double);
-    }
-    const size_t copies = precision_mode == HMX_FIT_FP64 ? 1 : 2;
+    const size_t copies = mode == HMX_FIT_FP64 ? 1 : 2;
     *bytes = copies * m * m * sizeof(double)             // A, then its factor U (+ a copy of A)
-             + x_bytes + 2 * nblk * kNB * kNB * sizeof(double)  // inverses of diagonal blocks
+             + fit_x_bytes(n_data, n_src, mode)        // the scaled jacobian (or its slices)
+             + 2 * nblk * kNB * kNB * sizeof(double)  // inverses of diagonal blocks
              + (10 * m + n + 256) * sizeof(double)  // 10 m-vectors, n-vector, partials
              + split_doubles(m > n ? m : n) * sizeof(double) + 256;  // split sums, info
     return HMX_OK;
@@ -1585,8 +1639,14 @@ int hmx_eqs_fit(const double *easting, const double *northing, const double *upw
     if (st) return st;
     if (!workspace || workspace_bytes < need)
         return fail(HMX_ERR_ARG, "workspace too small: %zu < %zu bytes", workspace_bytes, need);
+    // damping=None: LinearRegression's minimum-norm least squares
+    const bool lstsq = (precision_mode & HMX_FIT_NO_DAMPING) != 0;
+    precision_mode &= ~HMX_FIT_NO_DAMPING;
+    if (lstsq && damping != 0.0)
+        return fail(HMX_ERR_ARG, "damping must be 0 with HMX_FIT_NO_DAMPING");
     hipStream_t s = (hipStream_t)stream;
     const int64_t m = n_src, nd = n_data;
+    const bool svd_ok = svd_route_fits(nd, m);
     // mixed: a low-precision normal matrix preconditions CG on the FP64 one
     const bool mixed = precision_mode != HMX_FIT_FP64;
     const bool int8 = precision_mode == HMX_FIT_INT8_SPLIT;
@@ -1619,13 +1679,8 @@ int hmx_eqs_fit(const double *easting, const double *northing, const double *upw
     }
     void *X = ws;
     double *fgroup = nullptr;  // split: (n_fold, m) power-of-2 column scales per row group
-    const int64_t n_fold = (n_groups * unit_rows + kFoldRows - 1) / kFoldRows;
-    if (split) {
-        ws += (size_t)kSlices * n_groups * m * 16;
-        fgroup = reinterpret_cast<double *>(ws);
-        ws += (size_t)n_fold * m * sizeof(double);
-    } else
-        ws += (size_t)nd * m * (mixed ? sizeof(float) : sizeof(double));
+    if (split) fgroup = reinterpret_cast<double *>(ws + (size_t)kSlices * n_groups * m * 16);
+    ws += fit_x_bytes(nd, m, precision_mode);
     f.Dinv = reinterpret_cast<double *>(ws);
     f.DinvT = f.Dinv + nblk * kNB * kNB;
     double *vec = f.DinvT + nblk * kNB * kNB;
@@ -1695,6 +1750,9 @@ int hmx_eqs_fit(const double *easting, const double *northing, const double *upw
         return f.reduce(1, wk, nullptr, m, 1, out);
     };
     int pivot = 0, attempts = 0;
+    // The SVD route: the reference's Ridge falls back to an SVD when its
+    // Cholesky solve fails, and its damping=None fit is an SVD solve.
+    bool use_svd = false;
     for (;;) {
         if (mixed && attempts > 0) {  // restore the normal matrix, add the shift
             HMX_HIP(hipMemcpyAsync(f.A, A_copy, (size_t)m * m * sizeof(double),
@@ -1741,6 +1799,10 @@ int hmx_eqs_fit(const double *easting, const double *northing, const double *upw
         st = f.cholesky(&pivot);
         if (st) return st;
         if (pivot == 0) break;
+        if (svd_ok) {
+            use_svd = true;
+            break;
+        }
         if (++attempts > 12)
             return fail(HMX_ERR_NUMERIC, "normal matrix not positive definite (column %d)", pivot);
         if (dmax == 0.0) {
@@ -1757,11 +1819,24 @@ int hmx_eqs_fit(const double *easting, const double *northing, const double *upw
         tau = tau > 0.0 ? 10.0 * tau : tau0;
     }
 
+    // An unregularised fit whose factor has a small pivot ratio may have
+    // singular values below lstsq's cut-off: SVD route.
+    if (lstsq && svd_ok && !use_svd) {
+        double umin = 0.0, umax = 0.0;
+        st = f.reduce(2, f.A, nullptr, m, m + 1, &umin);
+        if (st) return st;
+        st = f.reduce(1, f.A, nullptr, m, m + 1, &umax);
+        if (st) return st;
+        if (!(umin * umin >= kLstsqPivotRatio * umax * umax)) use_svd = true;
+    }
+
     // 3. x = (factor)^-1 b.
-    hipLaunchKernelGGL(copy_kernel, dim3(gm), dim3(256), 0, s, m, b, wk);
-    HMX_LAUNCHED("copy_kernel");
-    st = f.solve(wk, wk2, x);
-    if (st) return st;
+    if (!use_svd) {
+        hipLaunchKernelGGL(copy_kernel, dim3(gm), dim3(256), 0, s, m, b, wk);
+        HMX_LAUNCHED("copy_kernel");
+        st = f.solve(wk, wk2, x);
+        if (st) return st;
+    }
 
     // 4. Preconditioned CG on the exact FP64 operator A = S J^T W J S + alpha I
     // (starting from x), stopping when the data-space update
@@ -1771,7 +1846,7 @@ int hmx_eqs_fit(const double *easting, const double *northing, const double *upw
     // A shifted factorisation is only a preconditioner: refine even when
     // the caller asked for a direct solve.
     const int allowed = (tau > 0.0 && max_iter == 0) ? 200 : max_iter;
-    if (allowed > 0) {
+    if (allowed > 0 && !use_svd) {
         double *t_dat = f.tmp;
         st = f.apply(x, t_dat, q);  // q = A x
         if (st) return st;
@@ -1817,6 +1892,51 @@ int hmx_eqs_fit(const double *easting, const double *northing, const double *upw
             rz = rz_new;
         }
         if (iters > allowed) iters = allowed;
+        // an unregularised fit whose refinement did not converge
+        if (lstsq && svd_ok && last_rel > rtol) use_svd = true;
+    }
+
+    // 4b. The SVD route on the scaled, weighted jacobian X = W^1/2 J S:
+    // with n_data >= m its column-major copy (over the normal matrix's copy
+    // and the jacobian region), else its row-major copy (X^T column-major,
+    // the FP64 mode's own X); the rotations over the normal matrix;
+    // x = V diag(f(s)) U^T W^1/2 d with the filter of the reference's
+    // solver for this problem.
+    int sweeps = 0, rank = 0;
+    if (use_svd) {
+        double *G = mixed ? A_copy : static_cast<double *>(X);
+        const bool transposed = m > nd;
+        const unsigned gyc = (unsigned)(m < 512 ? m : 512);
+        if (!transposed) {
+            hipLaunchKernelGGL(scaled_jacobian_cm_kernel, dim3(blocks_for(nd), gyc), dim3(256), 0,
+                               s, easting, northing, upward, weights, nd, p_easting, p_northing,
+                               p_upward, f.scale, m, G);
+            HMX_LAUNCHED("scaled_jacobian_cm_kernel");
+        } else if (mixed) {
+            hipLaunchKernelGGL(scaled_jacobian_kernel<double>, dim3(gm, gy), dim3(256), 0, s,
+                               easting, northing, upward, weights, nd, p_easting, p_northing,
+                               p_upward, f.scale, m, G, m);
+            HMX_LAUNCHED("scaled_jacobian_kernel");
+        }
+        hipLaunchKernelGGL(sqrtw_times_kernel, dim3(blocks_for(nd)), dim3(256), 0, s, nd, weights,
+                           data, f.tmp);
+        HMX_LAUNCHED("sqrtw_times_kernel");
+        const double eps = std::ldexp(1.0, -52);
+        int filter = HMX_SVD_RIDGE;
+        double cutoff = 1e-15;  // sklearn's _solve_svd
+        if (lstsq) {
+            filter = HMX_SVD_LSTSQ;  // LinearRegression: lstsq(cond = max(n, m) eps)
+            cutoff = (double)(nd > m ? nd : m) * eps;
+        } else if (transposed) {
+            filter = HMX_SVD_DUAL_LSTSQ;  // Ridge's dual form: lstsq(X X^T + alpha I)
+            cutoff = eps;
+        }
+        st = svd_lsq(G, transposed ? m : nd, transposed ? nd : m, transposed, f.tmp,
+                     lstsq ? 0.0 : damping, filter, cutoff, f.A, r, f.dinfo + 1, x,
+                     kSvdMaxSweeps, &sweeps, &rank, s);
+        if (st) return st;
+        iters = 0;
+        last_rel = 0.0;
     }
 
     // 5. c = S c'.
@@ -1828,6 +1948,9 @@ int hmx_eqs_fit(const double *easting, const double *northing, const double *upw
     info[2] = tau;
     info[3] = (double)attempts;
     info[4] = dmax;
+    info[5] = use_svd ? 1.0 : 0.0;
+    info[6] = (double)sweeps;
+    info[7] = (double)rank;
     return HMX_OK;
 }
 

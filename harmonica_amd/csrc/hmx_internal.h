@@ -66,4 +66,19 @@ inline int64_t forward_chunks(int64_t n_obs, int64_t n_src, int n_cu) {
 
 inline int popcount10(uint32_t m) { return __builtin_popcount(m & HMX_ALL_FIELDS); }
 
+// Least squares min |G c - y|^2 (+ alpha |c|^2, per filter) through the
+// one-sided Jacobi SVD (lsq_svd.hip) of A (L x k, column-major, L >= k,
+// overwritten): A = G (n x m, transposed = 0) or A = G^T (transposed = 1).
+// W (k x k), vec (3 k doubles), counters (2 ints); y has n entries and
+// coef m.  Returns the sweeps taken (negative if max_sweeps stopped them)
+// and the number of kept components.  Synchronous.
+int svd_lsq(double *A, int64_t L, int64_t k, int transposed, const double *y, double alpha,
+            int filter, double cutoff, double *W, double *vec, int *counters, double *coef,
+            int max_sweeps, int *sweeps, int *rank, hipStream_t s);
+constexpr int kSvdMaxSweeps = 60;
+// Largest fits that take the SVD route when the Cholesky route cannot match
+// the reference's solver: at most this many sources and matrix entries.
+constexpr int64_t kSvdMaxSrc = 8192;
+constexpr int64_t kSvdMaxElems = (int64_t)1 << 27;
+
 }  // namespace hmx

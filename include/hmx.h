@@ -77,6 +77,17 @@ extern "C" {
                                 Cholesky, FP64-residual CG */
 #define HMX_FIT_INT8_SPLIT 3 /* J^T W J from exact int8 slice products on int8 MFMA,
                                 FP64 Cholesky, FP64-residual CG */
+/* Flag OR'ed into the precision mode of hmx_eqs_fit / hmx_eqs_gb_fit:
+ * damping=None, i.e. sklearn LinearRegression (scipy.linalg.lstsq with the
+ * cut-off max(n, m) * 2^-52) instead of Ridge(alpha=damping); damping must
+ * then be 0. */
+#define HMX_FIT_NO_DAMPING 0x100
+
+/* Filters of the singular values in hmx_lsq_svd (scikit-learn / SciPy
+ * semantics, see hmx_lsq_svd). */
+#define HMX_SVD_RIDGE 0       /* s > cutoff (absolute): s / (s^2 + alpha) */
+#define HMX_SVD_LSTSQ 1       /* s > cutoff * s_max: 1 / s */
+#define HMX_SVD_DUAL_LSTSQ 2  /* s^2 + alpha > cutoff * (s_max^2 + alpha): s / (s^2 + alpha) */
 
 /* Last error message of the calling thread ("" if none). */
 const char *hmx_last_error(void);
@@ -301,9 +312,18 @@ int hmx_eqs_fit_workspace_bytes(int64_t n_data, int64_t n_src, int precision_mod
  * (src/harmonica/_equivalent_sources/cartesian.py:279-280): the jacobian
  * (utils.py:54-74) followed by verde.base.least_squares (third-party, verde
  * >= 1.8.1): column scaling by the population standard deviation of J
- * (sklearn StandardScaler(with_mean=False)), weighted ridge normal
- * equations (sklearn Ridge(alpha=damping, fit_intercept=False); damping 0
- * stands for damping=None), unscaling of the coefficients.
+ * (sklearn StandardScaler(with_mean=False)), the weighted least-squares
+ * solve (sklearn Ridge(alpha=damping, fit_intercept=False), or with
+ * HMX_FIT_NO_DAMPING in precision_mode LinearRegression(fit_intercept=False)
+ * for damping=None), unscaling of the coefficients.  The solve follows
+ * scikit-learn's routes: the Cholesky factorisation of the normal
+ * equations; when it fails, and for damping=None when the factor's pivots
+ * or the refinement show an ill-conditioned jacobian, the SVD solve of
+ * hmx_lsq_svd on the scaled, weighted jacobian (Ridge's SVD fallback with
+ * its 1e-15 cut-off, or its dual-form fallback when n_src > n_data;
+ * LinearRegression's lstsq cut-off max(n, m) 2^-52).  The SVD route
+ * serves fits of at most 8192 sources and 2^27 jacobian entries; larger
+ * fits whose factorisation fails are shifted and refined by CG instead.
  *   data, weights (may be NULL): (n_data) device arrays;
  *   precision_mode: HMX_FIT_FP64 (FP64 MFMA normal matrix, FP64 Cholesky,
  *     direct solve), HMX_FIT_FP32_MFMA (FP32 Jacobian and FP32-MFMA normal
@@ -318,8 +338,10 @@ int hmx_eqs_fit_workspace_bytes(int64_t n_data, int64_t n_src, int precision_mod
  *   max_iter, rtol: CG limit and stop rule (relative change of the
  *     data-space prediction per iteration);
  *   coefs: (n_src) device output;
- *   info: HOST array of 5 doubles: [CG iterations, last relative update,
- *     shift tau, factorisation retries, max diagonal of the normal matrix].
+ *   info: HOST array of 8 doubles: [CG iterations, last relative update,
+ *     shift tau, factorisation retries, max diagonal of the normal matrix,
+ *     1 if the SVD route solved the fit, its Jacobi sweeps (negative if
+ *     the sweep limit stopped them), its kept singular values].
  * Synchronous: returns after the coefficients are written. */
 int hmx_eqs_fit(const double *easting, const double *northing, const double *upward,
                 int64_t n_data, const double *data, const double *weights,
@@ -327,6 +349,33 @@ int hmx_eqs_fit(const double *easting, const double *northing, const double *upw
                 int64_t n_src, double damping, int precision_mode, int max_iter, double rtol,
                 double *coefs, double *info, void *workspace, size_t workspace_bytes,
                 void *stream);
+
+/* Bytes of workspace needed by hmx_lsq_svd for an n x m matrix (the k x k
+ * rotation matrix, k = min(n, m), the transpose of G when m > n, three
+ * k-vectors and a few scalars). */
+int hmx_lsq_svd_workspace_bytes(int64_t n, int64_t m, size_t *bytes);
+
+/* Replaces the SVD solves behind verde.base.least_squares
+ * (cartesian.py:280; third-party: scikit-learn 1.7
+ * sklearn/linear_model/_ridge.py _solve_svd, the fallback of Ridge when its
+ * Cholesky solve fails, and scipy.linalg.lstsq, used by LinearRegression
+ * and by Ridge's dual-form fallback):
+ *   coef = V diag(f(s)) U^T y,  G = U diag(s) V^T,
+ * with the filter f of HMX_SVD_RIDGE (s / (s^2 + damping) for s > cutoff,
+ * sklearn's cut-off 1e-15), HMX_SVD_LSTSQ (1 / s for s > cutoff * s_max) or
+ * HMX_SVD_DUAL_LSTSQ (s / (s^2 + damping) for s^2 + damping > cutoff *
+ * (s_max^2 + damping)).  One-sided Jacobi SVD on the GPU (of G, or of its
+ * transpose when m > n).
+ *   G: (n x m) column-major device matrix, OVERWRITTEN when n >= m;
+ *   y: (n) device vector; coef: (m) device output;
+ *   sigma: (m) device output of the min(n, m) singular values (unsorted),
+ *     then zeros, or NULL;
+ *   info: HOST array of 2 doubles: [Jacobi sweeps (negative if the sweep
+ *     limit stopped it), kept components].
+ * Synchronous. */
+int hmx_lsq_svd(double *G, int64_t n, int64_t m, const double *y, double damping, int filter,
+                double cutoff, double *coef, double *sigma, double *info, void *workspace,
+                size_t workspace_bytes, void *stream);
 
 /* Bytes of workspace needed by hmx_eqs_gb_fit for n_data data and windows
  * of at most max_window_data data and max_window_sources sources. */
@@ -350,8 +399,10 @@ int hmx_eqs_gb_fit_workspace_bytes(int64_t n_data, int64_t max_window_data,
  *   rmse: device output of n_windows + 1 values, the RMS of the residue
  *     before the first window and after each (rmse_per_iteration_, :250,
  *     :278);
- *   info: HOST array of 5 doubles: [total CG iterations, last relative
- *     update, largest shift, total factorisation retries, largest diagonal].
+ *   info: HOST array of 8 doubles: [total CG iterations, last relative
+ *     update, largest shift, total factorisation retries, largest diagonal,
+ *     windows solved by the SVD route, their total Jacobi sweeps, their
+ *     total kept singular values].
  * Indices outside [0, n_data) or [0, n_src) give NaN, never an access out of
  * bounds.  Synchronous. */
 int hmx_eqs_gb_fit(const double *easting, const double *northing, const double *upward,

@@ -219,30 +219,67 @@ def eqs_sph_predict(coordinates, points, coefs, n_threads=0):
 
 
 def eqs_fit_coefs(jacobian, data, weights=None, damping=None):
-    """Restatement of verde's ``least_squares`` as called at cartesian.py:280:
-    columns scaled by their population standard deviation (zero -> 1), then
-    weighted ridge normal equations (damping None -> plain least squares),
-    then unscaled.  Pinned in the tests against scikit-learn's
-    StandardScaler(with_mean=False) + Ridge(fit_intercept=False)."""
+    """Restatement of verde's ``least_squares`` as called at cartesian.py:280
+    (verde 1.8, not vendored): columns scaled by their population standard
+    deviation (zero -> 1; StandardScaler(with_mean=False)), rows and data
+    weighted by sqrt(w), then scikit-learn's solver (1.7, not vendored):
+
+    * damping None: LinearRegression -> scipy.linalg.lstsq with the cut-off
+      cond = max(n, m) eps (minimum-norm solution, singular values
+      s <= cond s_max dropped);
+    * damping alpha, m <= n: Ridge's Cholesky solve of X^T X + alpha I
+      (scipy solve, assume_a="pos"); when that raises LinAlgError, the SVD
+      solve c = V diag(s / (s^2 + alpha)) U^T y over s > 1e-15;
+    * damping alpha, m > n: the dual form X^T (X X^T + alpha I)^-1 y, and
+      when its Cholesky solve raises, the dual from scipy.linalg.lstsq of
+      X X^T + alpha I (default cut-off eps).
+
+    The result is unscaled (c / scale).  Pinned in the tests against
+    scikit-learn's StandardScaler + Ridge / LinearRegression, including
+    singular problems."""
+    return eqs_fit_coefs_info(jacobian, data, weights, damping)[0]
+
+
+def eqs_fit_coefs_info(jacobian, data, weights=None, damping=None):
+    """``eqs_fit_coefs`` plus the route taken: "lstsq", "cholesky", "svd",
+    "dual" or "dual-lstsq"."""
+    import scipy.linalg
+
     J = np.asarray(jacobian, dtype=np.float64)
     d = np.asarray(data, dtype=np.float64).ravel()
     scale = J.std(axis=0)
     scale[scale == 0] = 1.0
     Js = J / scale
-    if weights is None:
-        w = np.ones_like(d)
-    else:
-        w = np.asarray(weights, dtype=np.float64).ravel()
+    sw = None if weights is None else np.sqrt(np.asarray(weights, dtype=np.float64).ravel())
+    X = Js if sw is None else Js * sw[:, None]
+    y = d if sw is None else d * sw
+    n, m = X.shape
     if damping is None:
-        sw = np.sqrt(w)
-        sol = np.linalg.lstsq(Js * sw[:, None], d * sw, rcond=None)[0]
+        cond = max(n, m) * np.finfo(np.float64).eps
+        sol = scipy.linalg.lstsq(X, y, cond=cond)[0]
+        route = "lstsq"
+    elif m > n:
+        K = X @ X.T + damping * np.eye(n)
+        try:
+            dual = scipy.linalg.solve(K, y, assume_a="pos")
+            route = "dual"
+        except np.linalg.LinAlgError:
+            dual = scipy.linalg.lstsq(K, y)[0]
+            route = "dual-lstsq"
+        sol = X.T @ dual
     else:
-        A = Js.T @ (Js * w[:, None]) + damping * np.eye(J.shape[1])
-        b = Js.T @ (w * d)
-        import scipy.linalg
-
-        sol = scipy.linalg.solve(A, b, assume_a="pos")
-    return sol / scale
+        A = X.T @ X + damping * np.eye(m)
+        try:
+            sol = scipy.linalg.solve(A, X.T @ y, assume_a="pos")
+            route = "cholesky"
+        except np.linalg.LinAlgError:
+            U, s, Vt = np.linalg.svd(X, full_matrices=False)
+            keep = s > 1e-15
+            f = np.zeros_like(s)
+            f[keep] = s[keep] / (s[keep] ** 2 + damping)
+            sol = Vt.T @ (f * (U.T @ y))
+            route = "svd"
+    return sol / scale, route
 
 
 MAGNETIC_FIELDS = ("b", "b_e", "b_n", "b_u")

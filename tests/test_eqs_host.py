@@ -120,7 +120,10 @@ def test_fit_workspace_bytes_per_mode():
     holds the FP64 scaled jacobian, the FP32 mode an FP32 one plus a copy of
     the normal matrix, the split modes four slices per entry (2 bytes each
     for bf16, 1 for int8, rows padded to 32 or 64) and one scale per column
-    and group of 1024 rows; unknown modes are rejected."""
+    and group of 1024 rows.  Fits small enough for the SVD route (at most
+    8192 sources and 2^27 jacobian entries) get at least the FP64 jacobian
+    behind the normal matrix's copy; the damping=None flag does not change
+    the size; unknown modes are rejected."""
     import ctypes
 
     from harmonica_amd import _lib
@@ -132,16 +135,27 @@ def test_fit_workspace_bytes_per_mode():
         st = L.hmx_eqs_fit_workspace_bytes(n, m, mode, ctypes.byref(b))
         return st, b.value
 
+    def region(n, m, x_bytes, svd=True):
+        return max(x_bytes, n * m * 8 - m * m * 8) if svd else x_bytes
+
     n, m = 5000, 700
     st, b64 = nbytes(n, m, _lib.HMX_FIT_FP64)
     assert st == _lib.HMX_OK
     common = b64 - n * m * 8
     st, b32 = nbytes(n, m, _lib.HMX_FIT_FP32_MFMA)
-    assert st == 0 and b32 == common + m * m * 8 + n * m * 4
+    assert st == 0 and b32 == common + m * m * 8 + region(n, m, n * m * 4)
     n_fold = 5
     st, bb = nbytes(n, m, _lib.HMX_FIT_BF16_SPLIT)
-    assert st == 0 and bb == common + m * m * 8 + 4 * 5024 * m * 2 + n_fold * m * 8
+    assert st == 0 and bb == common + m * m * 8 + region(n, m, 4 * 5024 * m * 2 + n_fold * m * 8)
     st, bi = nbytes(n, m, _lib.HMX_FIT_INT8_SPLIT)
-    assert st == 0 and bi == common + m * m * 8 + 4 * 5056 * m + n_fold * m * 8
+    assert st == 0 and bi == common + m * m * 8 + region(n, m, 4 * 5056 * m + n_fold * m * 8)
+    for mode, b in ((_lib.HMX_FIT_FP64, b64), (_lib.HMX_FIT_INT8_SPLIT, bi)):
+        assert nbytes(n, m, mode | _lib.HMX_FIT_NO_DAMPING) == (0, b)
+    # beyond the SVD route's sizes the mode's own jacobian only
+    n, m = 20000, 7000
+    st, b64 = nbytes(n, m, _lib.HMX_FIT_FP64)
+    common = b64 - n * m * 8
+    st, bi = nbytes(n, m, _lib.HMX_FIT_INT8_SPLIT)
+    assert st == 0 and bi == common + m * m * 8 + 4 * (20032 // 16) * 16 * m + 20 * m * 8
     assert nbytes(n, m, 4)[0] == _lib.HMX_ERR_ARG
     assert nbytes(0, m, _lib.HMX_FIT_FP64)[0] == _lib.HMX_ERR_ARG

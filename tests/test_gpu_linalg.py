@@ -195,3 +195,80 @@ def test_cholesky_reports_indefinite(env, m, bad):
     _lib.check(L.hmx_cholesky_solve(_lib.ptr(dA), m, _lib.ptr(dx), ctypes.byref(info), _lib.ptr(ws),
                                     nbytes.value, None))
     assert info.value == bad + 1
+
+
+def _lsq_svd(env, G, y, damping, filt, cutoff):
+    """hmx_lsq_svd on a host matrix: (coef, singular values, info)."""
+    torch, _lib, L = env
+    n, m = G.shape
+    g = torch.tensor(np.asfortranarray(G).ravel(order="F")).cuda()
+    yd = torch.tensor(y).cuda()
+    coef = torch.empty(m, dtype=torch.float64, device="cuda")
+    sig = torch.empty(m, dtype=torch.float64, device="cuda")
+    nbytes = ctypes.c_size_t(0)
+    _lib.check(L.hmx_lsq_svd_workspace_bytes(n, m, ctypes.byref(nbytes)))
+    ws = torch.empty(nbytes.value, dtype=torch.uint8, device="cuda")
+    info = (ctypes.c_double * 2)()
+    _lib.check(L.hmx_lsq_svd(_lib.ptr(g), n, m, _lib.ptr(yd), damping, filt, cutoff,
+                             _lib.ptr(coef), _lib.ptr(sig), info, _lib.ptr(ws), nbytes.value,
+                             None))
+    return coef.cpu().numpy(), sig.cpu().numpy(), list(info)
+
+
+def _filtered_solution(G, y, damping, filt, cutoff):
+    U, s, Vt = np.linalg.svd(G, full_matrices=False)
+    f = np.zeros_like(s)
+    if filt == 0:
+        keep = s > cutoff
+        f[keep] = s[keep] / (s[keep] ** 2 + damping)
+    elif filt == 1:
+        keep = s > cutoff * s[0]
+        f[keep] = 1 / s[keep]
+    else:
+        keep = s ** 2 + damping > cutoff * (s[0] ** 2 + damping)
+        f[keep] = s[keep] / (s[keep] ** 2 + damping)
+    return Vt.T @ (f * (U.T @ y)), s, int(keep.sum())
+
+
+@pytest.mark.parametrize("shape", [(40, 40), (257, 130), (90, 201), (1, 3), (5, 1)])
+@pytest.mark.parametrize("filt", [0, 1, 2])
+def test_lsq_svd_random(env, shape, filt):
+    """Well-conditioned random matrices (tall, square, wide, degenerate
+    shapes): singular values against LAPACK's (sorted) and the filtered
+    solution of each filter against the same formula on numpy's SVD."""
+    rng = np.random.default_rng(shape[0] * 1000 + shape[1] + filt)
+    G = rng.normal(size=shape)
+    y = rng.normal(size=shape[0])
+    damping, cutoff = {0: (0.3, 1e-15), 1: (0.0, 1e-12), 2: (0.3, 2.0 ** -52)}[filt]
+    coef, sig, info = _lsq_svd(env, G, y, damping, filt, cutoff)
+    expected, s, rank = _filtered_solution(G, y, damping, filt, cutoff)
+    assert info[0] > 0 and int(info[1]) == rank
+    k = min(shape)
+    npt.assert_allclose(np.sort(sig)[::-1][:k], s, rtol=1e-12, atol=1e-13 * s[0])
+    if shape[1] > shape[0]:  # the extra columns are rotated to zero
+        assert np.sort(sig)[::-1][k:].max(initial=0) <= 1e-13 * s[0]
+    npt.assert_allclose(coef, expected, rtol=1e-10, atol=1e-11 * np.abs(expected).max())
+
+
+def test_lsq_svd_graded_and_rank_deficient(env):
+    """Singular values spread over 12 decades (recovered to roundoff of the
+    largest) and three exactly repeated columns: the lstsq filter drops the
+    repeated directions to give the minimum-norm solution (repeated columns
+    share the weight)."""
+    rng = np.random.default_rng(3)
+    n, m = 150, 60
+    Q1, _ = np.linalg.qr(rng.normal(size=(n, m)))
+    Q2, _ = np.linalg.qr(rng.normal(size=(m, m)))
+    s_true = np.logspace(0, -12, m)
+    G = (Q1 * s_true) @ Q2.T
+    coef, sig, info = _lsq_svd(env, G.copy(), rng.normal(size=n), 0.0, 1, 1e-15)
+    npt.assert_allclose(np.sort(sig)[::-1], s_true, rtol=1e-9, atol=1e-14)
+    G = (Q1 * np.logspace(0, -3, m)) @ Q2.T
+    H = np.concatenate([G[:, :40], G[:, :3]], axis=1)
+    y = rng.normal(size=n)
+    coef, sig, info = _lsq_svd(env, H, y, 0.0, 1, max(H.shape) * 2.0 ** -52)
+    expected = np.linalg.lstsq(H, y, rcond=max(H.shape) * 2.0 ** -52)[0]
+    assert int(info[1]) == 40
+    npt.assert_allclose(H @ coef, H @ expected, rtol=0, atol=1e-10 * np.abs(H @ expected).max())
+    npt.assert_allclose(coef[:3], coef[40:], rtol=1e-8)
+    npt.assert_allclose(coef, expected, rtol=1e-6, atol=1e-8 * np.abs(expected).max())

@@ -186,3 +186,72 @@ def test_eqs_fit_restatement_matches_sklearn(damping, weighted):
     npt.assert_allclose(J @ got, J @ expected, rtol=1e-6, atol=1e-9 * np.abs(J @ expected).max())
     if damping is not None:
         npt.assert_allclose(got, expected, rtol=1e-6, atol=1e-8 * np.abs(expected).max())
+
+
+def _deep_sources_problem(depth, n_side=20, spacing=51.0, wide=False):
+    """Sources `depth` below a 20 x 20 grid of 51 m spacing (one per datum,
+    or two per datum when ``wide``); data from two deeper point masses."""
+    e = np.arange(n_side) * spacing
+    E, N = np.meshgrid(e, e)
+    coords = (E.ravel(), N.ravel(), np.zeros(E.size))
+    if wide:
+        pts = (np.concatenate([coords[0], coords[0] + spacing / 3]),
+               np.concatenate([coords[1], coords[1] + spacing / 4]),
+               np.concatenate([coords[2] - depth, coords[2] - 1.3 * depth]))
+    else:
+        pts = (coords[0], coords[1], coords[2] - depth)
+    masses = (np.array([300.0, 700.0]), np.array([500.0, 200.0]), np.array([-800.0, -600.0]))
+    data = oracle.eqs_jacobian(coords, masses) @ np.array([1e6, -2e6])
+    return oracle.eqs_jacobian(coords, pts), data
+
+
+@pytest.mark.parametrize("depth,damping,route", [
+    (200.0, 1e-18, "cholesky"), (500.0, 1e-18, "svd"), (1000.0, 1e-18, "svd"),
+    (500.0, None, "lstsq"), (1000.0, None, "lstsq"), (1000.0, 1e-3, "cholesky"),
+])
+def test_eqs_fit_restatement_singular_matches_sklearn(depth, damping, route):
+    """Near-singular scaled jacobians (condition numbers 2e8 to 1e20): the
+    restatement takes scikit-learn's route (Cholesky, the SVD fallback of
+    Ridge when the Cholesky solve fails, lstsq's cut-off for
+    LinearRegression) and matches its coefficients."""
+    import warnings
+
+    from sklearn.linear_model import LinearRegression, Ridge
+    from sklearn.preprocessing import StandardScaler
+
+    J, data = _deep_sources_problem(depth)
+    got, taken = oracle.eqs_fit_coefs_info(J, data, None, damping)
+    assert taken == route
+    scaler = StandardScaler(copy=True, with_mean=False, with_std=True)
+    Js = scaler.fit_transform(J)
+    regr = LinearRegression(fit_intercept=False) if damping is None else Ridge(alpha=damping, fit_intercept=False)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        regr.fit(Js, data)
+    expected = regr.coef_ / scaler.scale_
+    npt.assert_allclose(J @ got, J @ expected, rtol=0, atol=1e-10 * np.abs(J @ expected).max())
+    npt.assert_allclose(got, expected, rtol=0, atol=1e-8 * np.abs(expected).max())
+
+
+@pytest.mark.parametrize("damping", [None, 1e-18, 1e-3])
+def test_eqs_fit_restatement_wide_matches_sklearn(damping):
+    """More sources than data (m > n): Ridge's dual form (and its lstsq
+    fallback) and LinearRegression's minimum-norm solution."""
+    import warnings
+
+    from sklearn.linear_model import LinearRegression, Ridge
+    from sklearn.preprocessing import StandardScaler
+
+    J, data = _deep_sources_problem(400.0, n_side=12, wide=True)
+    assert J.shape[1] > J.shape[0]
+    got, taken = oracle.eqs_fit_coefs_info(J, data, None, damping)
+    assert taken in (("lstsq",) if damping is None else ("dual", "dual-lstsq"))
+    scaler = StandardScaler(copy=True, with_mean=False, with_std=True)
+    Js = scaler.fit_transform(J)
+    regr = LinearRegression(fit_intercept=False) if damping is None else Ridge(alpha=damping, fit_intercept=False)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        regr.fit(Js, data)
+    expected = regr.coef_ / scaler.scale_
+    npt.assert_allclose(J @ got, J @ expected, rtol=0, atol=1e-10 * np.abs(J @ expected).max())
+    npt.assert_allclose(got, expected, rtol=0, atol=1e-8 * np.abs(expected).max())
