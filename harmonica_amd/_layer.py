@@ -17,6 +17,7 @@ import numpy as np
 
 from . import _lib
 from ._forward import FIELDS, layer_fields_device, progressbar_context, to_device
+from ._grid_cells import all_cells, one_cell, top_and_bottom
 
 
 class DataArrayLite(np.lib.mixins.NDArrayOperatorsMixin):
@@ -195,37 +196,9 @@ class DatasetAccessorPrismLayer:
     def shape(self):
         return (self._obj.northing.size, self._obj.easting.size)
 
-    def _get_prism_horizontal_boundaries(self, easting, northing):
-        spacing = self.spacing
-        west = easting - spacing[1] / 2
-        east = easting + spacing[1] / 2
-        south = northing - spacing[0] / 2
-        north = northing + spacing[0] / 2
-        return west, east, south, north
-
     def update_top_bottom(self, surface, reference):
         """layer.py:264-312."""
-        surface, reference = np.asarray(surface), np.asarray(reference)
-        if surface.shape != self.shape:
-            raise ValueError(
-                f"Invalid surface array with shape '{surface.shape}'. "
-                + "Its shape should be compatible with the coordinates "
-                + "of the layer of prisms."
-            )
-        if reference.ndim != 0:
-            if reference.shape != self.shape:
-                raise ValueError(
-                    f"Invalid reference array with shape '{reference.shape}'. "
-                    + "Its shape should be compatible with the coordinates "
-                    + "of the layer of prisms."
-                )
-        else:
-            reference = reference * np.ones(self.shape)
-        top = surface.copy()
-        bottom = reference.copy()
-        reverse = surface < reference
-        top[reverse] = reference[reverse]
-        bottom[reverse] = surface[reverse]
+        top, bottom = top_and_bottom(surface, reference, self.shape, "prisms")
         self._obj._set_coord("top", self.dims, top)
         self._obj._set_coord("bottom", self.dims, bottom)
 
@@ -277,23 +250,14 @@ class DatasetAccessorPrismLayer:
         return result.reshape(cast.shape)
 
     def _to_prisms(self):
-        easting, northing = np.meshgrid(self._obj.easting.values, self._obj.northing.values)
-        west, east, south, north = self._get_prism_horizontal_boundaries(
-            easting.ravel(), northing.ravel()
-        )
-        bottom = self._obj.bottom.values.ravel()
-        top = self._obj.top.values.ravel()
-        return np.vstack((west, east, south, north, bottom, top)).T
+        s_north, s_east = self.spacing
+        return all_cells(self._obj.easting.values, self._obj.northing.values, s_east, s_north,
+                         self._obj.bottom.values, self._obj.top.values)
 
     def get_prism(self, indices):
-        center_easting = self._obj.easting.values[indices[1]]
-        center_northing = self._obj.northing.values[indices[0]]
-        west, east, south, north = self._get_prism_horizontal_boundaries(
-            center_easting, center_northing
-        )
-        bottom = self._obj.bottom.values[indices]
-        top = self._obj.top.values[indices]
-        return west, east, south, north, bottom, top
+        s_north, s_east = self.spacing
+        return one_cell(self._obj.easting.values, self._obj.northing.values, s_east, s_north,
+                        self._obj.bottom.values, self._obj.top.values, tuple(indices))
 
     def to_pyvista(self, drop_null_prisms=True):
         """Export to pyvista (layer.py:485-519); needs pyvista."""

@@ -6,10 +6,9 @@ radii and physical properties, and the ``tesseroid_layer`` accessor whose
 ``gravity`` calls :func:`harmonica_amd.tesseroid_gravity` (the GPU path).
 """
 
-import warnings
-
 import numpy as np
 
+from ._grid_cells import all_cells, one_cell, present_cells, top_and_bottom
 from ._layer import DataArrayLite, _centres
 from ._tesseroid import tesseroid_gravity
 
@@ -152,27 +151,8 @@ class DatasetAccessorTesseroidLayer:
                 self._obj.latitude.values.max() + s_lat / 2)
 
     def update_top_bottom(self, surface, reference):
-        surface, reference = np.asarray(surface), np.asarray(reference)
-        if surface.shape != self.shape:
-            raise ValueError(
-                f"Invalid surface array with shape '{surface.shape}'. "
-                + "Its shape should be compatible with the coordinates "
-                + "of the layer of tesseroids."
-            )
-        if reference.ndim != 0:
-            if reference.shape != self.shape:
-                raise ValueError(
-                    f"Invalid reference array with shape '{reference.shape}'. "
-                    + "Its shape should be compatible with the coordinates "
-                    + "of the layer of tesseroids."
-                )
-        else:
-            reference = reference * np.ones(self.shape)
-        top = surface.copy()
-        bottom = reference.copy()
-        reverse = surface < reference
-        top[reverse] = reference[reverse]
-        bottom[reverse] = surface[reverse]
+        """tesseroid_layer.py:272-320."""
+        top, bottom = top_and_bottom(surface, reference, self.shape, "tesseroids")
         self._obj._set_coord("top", self.dims, top)
         self._obj._set_coord("bottom", self.dims, bottom)
 
@@ -194,33 +174,17 @@ class DatasetAccessorTesseroidLayer:
                                  field=field, progressbar=progressbar, **kwargs)
 
     def _get_nonans_mask(self, property_name=None):
-        mask = np.logical_and(np.logical_not(np.isnan(self._obj.top.values)),
-                              np.logical_not(np.isnan(self._obj.bottom.values)))
-        if property_name is not None:
-            mask_property = np.logical_not(np.isnan(self._obj[property_name].values))
-            if not mask_property[mask].all():
-                warnings.warn(
-                    f"Found missing values in '{property_name}' property "
-                    + "of the tesseroid layer. The tesseroids with nan as "
-                    + f"'{property_name}' will be ignored.",
-                    stacklevel=1,
-                )
-            mask = np.logical_and(mask, mask_property)
-        return mask
-
-    def _get_tesseroid_horizontal_boundaries(self, longitude, latitude):
-        s_lat, s_lon = self.spacing
-        return (longitude - s_lon / 2, longitude + s_lon / 2, latitude - s_lat / 2,
-                latitude + s_lat / 2)
+        """tesseroid_layer.py:406-441."""
+        values = None if property_name is None else self._obj[property_name].values
+        return present_cells(self._obj.top.values, self._obj.bottom.values, values,
+                             property_name)
 
     def _to_tesseroids(self):
-        longitude, latitude = np.meshgrid(self._obj.longitude.values, self._obj.latitude.values)
-        w, e, s, n = self._get_tesseroid_horizontal_boundaries(longitude.ravel(),
-                                                               latitude.ravel())
-        return np.vstack((w, e, s, n, self._obj.bottom.values.ravel(),
-                          self._obj.top.values.ravel())).T
+        s_lat, s_lon = self.spacing
+        return all_cells(self._obj.longitude.values, self._obj.latitude.values, s_lon, s_lat,
+                         self._obj.bottom.values, self._obj.top.values)
 
     def get_tesseroid(self, indices):
-        w, e, s, n = self._get_tesseroid_horizontal_boundaries(
-            self._obj.longitude.values[indices[1]], self._obj.latitude.values[indices[0]])
-        return w, e, s, n, self._obj.bottom.values[indices], self._obj.top.values[indices]
+        s_lat, s_lon = self.spacing
+        return one_cell(self._obj.longitude.values, self._obj.latitude.values, s_lon, s_lat,
+                        self._obj.bottom.values, self._obj.top.values, tuple(indices))
