@@ -54,6 +54,8 @@ SIGNATURES = {
     "hmx_forward_workspace_bytes": (_int, [_i64, _i64, _u32, _szp]),
     "hmx_prism_gravity": (_int, [_vp, _vp, _vp, _i64, _vp, _vp, _i64, _u32, _vp, _vp, _vp,
                                  ctypes.c_size_t, _vp]),
+    "hmx_prism_gravity_static": (_int, [_vp, _vp, _vp, _i64, _vp, _vp, _i64, _u32, _vp, _vp,
+                                        _vp]),
     "hmx_prism_singular_scan": (_int, [_vp, _vp, _vp, _i64, _vp, _i64, _u32, _vp, _vp]),
     "hmx_prism_layer_gravity": (_int, [_vp, _vp, _vp, _i64, _vp, _i64, _vp, _i64, _vp, _vp, _vp,
                                        _dbl, _u32, _vp, _vp, _vp, ctypes.c_size_t, _vp]),

@@ -9,7 +9,7 @@
 namespace {
 thread_local char g_last_error[1024] = "";
 
-constexpr int kMaxDevices = 64;
+using hmx::kMaxDevices;
 int g_cu_count[kMaxDevices] = {0};
 std::mutex g_mutex;
 }  // namespace

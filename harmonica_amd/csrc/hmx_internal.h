@@ -17,6 +17,9 @@ int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
 // Clear the calling thread's last error.
 void clear_error();
 
+// Devices the per-device caches of the library cover.
+constexpr int kMaxDevices = 64;
+
 // Number of compute units of the current device (cached per device), or a
 // negative status if no device is usable.
 int device_cu_count();

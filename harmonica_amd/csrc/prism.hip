@@ -7,6 +7,8 @@
 // evaluates the same prism against its own point.  Each chunk writes its
 // partial sums to the workspace; a second kernel reduces the chunks in
 // order and applies harmonica's units, so results are deterministic.
+#include <mutex>
+
 #include "hmx_internal.h"
 #include "prism_device.h"
 
@@ -259,15 +261,19 @@ bool has_kernel(uint32_t mask) {
 
 // Run the forward model for any field mask: one launch when the mask has
 // its own kernel, else one launch per field reusing the workspace.
+// chunk_cap > 0 limits the number of source chunks (so that the partial
+// sums fit a fixed scratch buffer).
 template <class Src>
 int run_forward(const double *oe, const double *on, const double *ou, int64_t n_obs,
                 const Src &src, int64_t n_src, uint32_t mask, double *out, uint32_t *flags,
-                void *workspace, size_t workspace_bytes, hipStream_t stream) {
+                void *workspace, size_t workspace_bytes, hipStream_t stream,
+                int64_t chunk_cap = 0) {
     if (n_obs <= 0) return HMX_OK;
     int cu = device_cu_count();
     if (cu < 0) return cu;
     const int nf = popcount10(mask);
-    const int64_t chunks = forward_chunks(n_obs, n_src, cu);
+    int64_t chunks = forward_chunks(n_obs, n_src, cu);
+    if (chunk_cap > 0 && chunks > chunk_cap) chunks = chunk_cap;
     if (n_src == 0) {
         // No sources: the reference returns zeros.
         int64_t total = (int64_t)nf * n_obs;
@@ -312,10 +318,69 @@ int check_common(int64_t n_obs, int64_t n_src, uint32_t mask, const void *out) {
     return HMX_OK;
 }
 
+// Scratch of hmx_prism_gravity_static: the chunk partial sums, in a
+// file-scope device array (one per device, held until the process ends).
+// Host code uses its per-device address from hipGetSymbolAddress, never the
+// array's name.
+constexpr int64_t kStaticScratchDoubles = (int64_t)1 << 22;  // 32 MiB
+__device__ __align__(256) double scratch[kStaticScratchDoubles];
+
+std::mutex g_scratch_mutex;
+double *g_scratch_ptr[kMaxDevices] = {};
+
+// Address of the current device's scratch array (looked up once per
+// device, with that device current).
+int static_scratch(double **out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return fail(HMX_ERR_NODEV, "no HIP device: %s", hipGetErrorString(e));
+    if (dev < 0 || dev >= kMaxDevices) return fail(HMX_ERR_NODEV, "device index %d out of range", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(HMX_ERR_HIP, "hipGetSymbolAddress(scratch) failed on device %d: %s", dev,
+                        hipGetErrorString(e));
+        g_scratch_ptr[dev] = static_cast<double *>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return HMX_OK;
+}
+
 }  // namespace
 }  // namespace hmx
 
 extern "C" {
+
+int hmx_prism_gravity_static(const double *easting, const double *northing,
+                             const double *upward, int64_t n_obs, const double *prisms,
+                             const double *density, int64_t n_prisms, uint32_t field_mask,
+                             double *out, uint32_t *singular_flags, void *stream) {
+    using namespace hmx;
+    clear_error();
+    int st = check_common(n_obs, n_prisms, field_mask, out);
+    if (st) return st;
+    if (n_obs > 0 && (!easting || !northing || !upward))
+        return fail(HMX_ERR_ARG, "NULL coordinate array");
+    if (n_prisms > 0 && (!prisms || !density)) return fail(HMX_ERR_ARG, "NULL prism array");
+    // one launch holds the partial sums of all its fields for at least one
+    // chunk: n_obs x (fields of one launch) doubles
+    const int64_t per_chunk = n_obs * (has_kernel(field_mask) ? popcount10(field_mask) : 1);
+    if (per_chunk > kStaticScratchDoubles)
+        return fail(HMX_ERR_ARG,
+                    "hmx_prism_gravity_static: %lld observation points x fields exceed its scratch "
+                    "of %lld doubles (use hmx_prism_gravity with a workspace)",
+                    (long long)per_chunk, (long long)kStaticScratchDoubles);
+    if (n_obs == 0) return HMX_OK;
+    double *ws = nullptr;
+    st = static_scratch(&ws);
+    if (st) return st;
+    ExplicitSource src{prisms, density};
+    return run_forward(easting, northing, upward, n_obs, src, n_prisms, field_mask, out,
+                       singular_flags, ws, (size_t)kStaticScratchDoubles * sizeof(double),
+                       (hipStream_t)stream, kStaticScratchDoubles / per_chunk);
+}
 
 int hmx_prism_gravity(const double *easting, const double *northing, const double *upward,
                       int64_t n_obs, const double *prisms, const double *density,

@@ -122,6 +122,22 @@ int hmx_prism_gravity(const double *easting, const double *northing, const doubl
                       uint32_t *singular_flags, void *workspace, size_t workspace_bytes,
                       void *stream);
 
+/* No counterpart in the reference: hmx_prism_gravity for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid allocation).  Same arguments and
+ * results, without the workspace: the chunk partial sums live in a
+ * file-scope device array of 32 MiB (2^22 doubles) per GPU, held until the
+ * process ends.  n_obs times the fields of one launch (all of field_mask
+ * for the masks with their own kernel: one field, g_e/g_n/g_z, the six
+ * tensor terms, all ten; else 1) must not exceed 2^22, else HMX_ERR_ARG;
+ * larger sums over prisms use fewer source chunks to fit.  Allocates
+ * nothing.  Every call on one GPU shares its array: calls that overlap on
+ * one GPU (several host threads or streams) race. */
+int hmx_prism_gravity_static(const double *easting, const double *northing,
+                             const double *upward, int64_t n_obs, const double *prisms,
+                             const double *density, int64_t n_prisms, uint32_t field_mask,
+                             double *out, uint32_t *singular_flags, void *stream);
+
 /* Replaces the singular-point scans _any_singular_point_g_{ee,nn,zz,en,ez,nz}
  * of _check_singular_points (src/harmonica/_forward/prisms/gravity.py:239-449):
  * ORs bit f into *flags (device uint32, caller-zeroed) when some observation
