@@ -29,52 +29,31 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-# MI355X FP64 vector peak (AMD published: 256 CU x 128 FP64 flop/clk x 2.4 GHz).
-FP64_PEAK = 78.6e12
-
-# FP64 operations per prism-observation pair executed by the g_z kernel
-# (prism_forward_kernel<g_z>, grouped closed form): rocprofv3 counters
-# SQ_INSTS_VALU_{ADD,MUL,TRANS}_F64 + 2 x SQ_INSTS_VALU_FMA_F64, x 64 lanes,
-# divided by the pairs of the launch, on this workload
-# (profiles/prism_gz_terrain1M_counters.md).  The same profile gives 322.1
-# VALU instructions per pair, 14.27 of them FP64 transcendentals (v_rsq_f64,
-# v_rcp_f64; 16 per pair less the square roots of bottom vertices shared
-# with the previous prism).  A wave64 VALU instruction occupies a SIMD for 4 cycles and
-# an FP64 transcendental for 16 (measured on the MI355X), which bounds the
-# pair rate at 1024 SIMDs x 2.4 GHz x 64 lanes / (4 (VALU - TRANS) + 16
-# TRANS) cycles.  The per-vertex form of the same closed form (8 square
-# roots, 16 logarithms, 8 arctangents per pair) measured 1083.7 FP64
-# operations per pair; it is reported beside the executed count.
-FLOPS_PER_PAIR_GZ = 368.2
-VALU_INSTR_PER_PAIR_GZ = 322.1
-TRANS_PER_PAIR_GZ = 14.27
-PER_VERTEX_FLOPS_PER_PAIR_GZ = 1083.7
-# Same counters for the ten fields fused (prism_forward_kernel<all>).
-FLOPS_PER_PAIR_ALL = 953.6
-VALU_INSTR_PER_PAIR_ALL = 825.1
-TRANS_PER_PAIR_ALL = 32.0
-SIMD_CYCLES_PER_S = 1024 * 2.4e9
-# Same counters for prism_magnetic_kernel<b_e|b_n|b_u> and <b_u> on the 1M
-# terrain prisms with their magnetisation (lean tensor forms: one logarithm
-# per off-diagonal and one arctangent per diagonal term, K_uu by Laplace).
-FLOPS_PER_PAIR_B = 474.5
-VALU_INSTR_PER_PAIR_B = 424.1
-TRANS_PER_PAIR_B = 16.12
-FLOPS_PER_PAIR_BU = 314.4
-VALU_INSTR_PER_PAIR_BU = 276.8
-TRANS_PER_PAIR_BU = 12.53
+# Frozen per-pair operation counts of the forward kernels (rocprofv3
+# counters on these workloads; provenance in the JSON file and the profiles
+# it names).
+with open(os.path.join(ROOT, "bench", "flop_model.json")) as _fh:
+    FLOP_MODEL = json.load(_fh)
+FP64_PEAK = FLOP_MODEL["fp64_peak_flops"]
+SIMD_CYCLES_PER_S = FLOP_MODEL["simd_cycles_per_s"]
+_GZ, _ALL = FLOP_MODEL["prism_gz"], FLOP_MODEL["prism_all_ten"]
+_B, _BU = FLOP_MODEL["magnetic_b"], FLOP_MODEL["magnetic_b_u"]
+FLOPS_PER_PAIR_GZ, VALU_INSTR_PER_PAIR_GZ, TRANS_PER_PAIR_GZ = (
+    _GZ["flops_per_pair"], _GZ["valu_instr_per_pair"], _GZ["trans_per_pair"])
+PER_VERTEX_FLOPS_PER_PAIR_GZ = _GZ["per_vertex_flops_per_pair"]
+FLOPS_PER_PAIR_ALL, VALU_INSTR_PER_PAIR_ALL, TRANS_PER_PAIR_ALL = (
+    _ALL["flops_per_pair"], _ALL["valu_instr_per_pair"], _ALL["trans_per_pair"])
+FLOPS_PER_PAIR_B, VALU_INSTR_PER_PAIR_B, TRANS_PER_PAIR_B = (
+    _B["flops_per_pair"], _B["valu_instr_per_pair"], _B["trans_per_pair"])
+FLOPS_PER_PAIR_BU, VALU_INSTR_PER_PAIR_BU, TRANS_PER_PAIR_BU = (
+    _BU["flops_per_pair"], _BU["valu_instr_per_pair"], _BU["trans_per_pair"])
+TRAFFIC_BYTES_GZ = _GZ["traffic_bytes"]
 
 
 def valu_issue_frac(pair_rate, valu, trans):
     """Fraction of the SIMDs' issue cycles the pair rate needs."""
     cycles_per_pair_wave = 4.0 * (valu - trans) + 16.0 * trans
     return pair_rate * cycles_per_pair_wave / (64.0 * SIMD_CYCLES_PER_S)
-
-
-# HBM bytes per launch of the g_z kernel on this workload: FETCH_SIZE
-# 219639 KiB x 2 (a 1 GiB copy reads as 524308 KiB) + WRITE_SIZE 20480 KiB
-# x 1 (profiles/prism_gz_terrain1M_counters.md).
-TRAFFIC_BYTES_GZ = (219639.0 * 2 + 20480.0) * 1024
 
 
 def log(msg):
@@ -102,6 +81,40 @@ def cpu_baseline(w, prisms, density, gpu_gz):
                       f"{prisms.shape[0]} prisms = {pairs:.3e} pairs, g_z FP64, {dt:.1f} s",
             "host_cores_visible": cores,
             "max_rel_diff_vs_gpu": float(np.abs(got - ref).max() / np.abs(ref).max())}
+
+
+def ten_field_parity(w, prisms, density, gpu_all):
+    """All ten fields of the timed fused launch against the oracle at 81 of
+    the 256 x 256 points (rows and columns 0, 17, 51, 102, 130, 153, 204,
+    240, 255), max |gpu - cpu| / max |cpu| per field, over all 81 points
+    and over the 56 that do not lie on a vertical line through prism edges
+    (x and y both on edge planes: rows/columns 0, 51, 102, 153, 204, 255 of
+    this grid, where the tensor terms take their measure-zero conventions)."""
+    import oracle
+    from harmonica_amd._lib import FIELDS
+
+    idx = np.array([0, 17, 51, 102, 130, 153, 204, 240, 255])
+    coords = tuple(c[np.ix_(idx, idx)] for c in w["coordinates"])
+    edges_e = np.concatenate([w["easting"] - 50.0, w["easting"] + 50.0])
+    edges_n = np.concatenate([w["northing"] - 50.0, w["northing"] + 50.0])
+    on_line = np.isin(coords[0], edges_e) & np.isin(coords[1], edges_n)
+    shape = w["coordinates"][0].shape
+    t0 = time.perf_counter()
+    out = {"points": int(on_line.size), "on_edge_lines": int(on_line.sum())}
+    worst_all = worst_off = 0.0
+    for k, field in enumerate(FIELDS):
+        ref = oracle.prism_gravity(coords, prisms, density, field)
+        got = gpu_all[k].reshape(shape)[np.ix_(idx, idx)]
+        scale = np.abs(ref).max()
+        e_all = float(np.abs(got - ref).max() / scale)
+        e_off = float(np.abs(got - ref)[~on_line].max() / scale)
+        out[field] = {"all_points": e_all, "off_edge_lines": e_off}
+        worst_all, worst_off = max(worst_all, e_all), max(worst_off, e_off)
+    out["max_all_points"], out["max_off_edge_lines"] = worst_all, worst_off
+    out["cpu_s"] = time.perf_counter() - t0
+    log(f"ten fields vs oracle at 81 points: {worst_all:.2e} (all), {worst_off:.2e} "
+        f"(off edge lines)")
+    return out
 
 
 def _sync_time(fn):
@@ -335,10 +348,19 @@ def bench_eqs(args, modes=None):
         out[mode] = {"fit_s": dt, "predict_s": dtp, "info": eqs.fit_info_}
         log(f"eqs {mode}: fit {dt:.2f} s, predict {dtp:.3f} s, {eqs.fit_info_}")
     ref = preds.get("float64")
+    from harmonica_amd._eqs import MIXED_PREDICTION_BAR, MIXED_RTOL
+
     for mode in modes[1:] if ref is not None else ():
         out[mode]["max_rel_pred_diff_vs_fp64"] = float(np.abs(preds[mode] - ref).max() /
                                                        np.abs(ref).max())
+        out[mode]["prediction_bar"] = MIXED_PREDICTION_BAR
+        out[mode]["cg_rtol"] = MIXED_RTOL
         out[mode]["speedup_vs_fp64"] = out["float64"]["fit_s"] / out[mode]["fit_s"]
+        # Both fits factorise the same size of matrix in FP64: with the rest
+        # of the mixed fit free, the speedup would be the FP64 fit's time over
+        # its factorisation's.
+        fact = out["float64"]["info"].get("factorisation_s", 0.0)
+        out[mode]["speedup_ceiling"] = out["float64"]["fit_s"] / fact if fact > 0 else None
     if args.eqs_oracle:
         import oracle
 
@@ -481,7 +503,9 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--extra", action="store_true", help="also time the fused ten-field launch")
+    ap.add_argument("--no-ten", action="store_true",
+                    help="skip the fused ten-field launch (timed by default)")
+    ap.add_argument("--extra", action="store_true", help="accepted; the ten-field leg is default")
     ap.add_argument("--obs-side", type=int, default=256)
     ap.add_argument("--prism-side", type=int, default=1000)
     ap.add_argument("--layer", action="store_true", help="also time the layer accessor path")
@@ -594,17 +618,25 @@ def main():
                 "per_vertex_equivalent_tflops": value * PER_VERTEX_FLOPS_PER_PAIR_GZ / 1e12}
 
     extra = {}
-    if args.extra:
+    if not args.no_ten:
+        # The full-tensor workload: all ten fields in one fused launch.
         mask = (1 << 10) - 1
-        step10, _ = make_step(mask, 10)
-        ms10, _ = timed(step10, 1, 1)
+        step10, out10 = make_step(mask, 10)
+        steps10 = max(1, min(args.steps, 3))
+        ms10, _ = timed(step10, steps10, 1)
         rate10 = pairs / (ms10 * 1e-3)
-        extra["all_ten_fields"] = {"ms_per_step": ms10, "pair_evals_per_s": rate10,
-                                   "fp64_tflops": rate10 * FLOPS_PER_PAIR_ALL / 1e12,
-                                   "fp64_peak_frac": rate10 * FLOPS_PER_PAIR_ALL / FP64_PEAK,
-                                   "valu_issue_frac": valu_issue_frac(rate10, VALU_INSTR_PER_PAIR_ALL,
-                                                                      TRANS_PER_PAIR_ALL)}
-        log(f"ten fields fused: {ms10:.2f} ms/step")
+        ach10 = rate10 * FLOPS_PER_PAIR_ALL
+        extra["all_ten_fields"] = {
+            "ms_per_step": ms10, "steps": steps10, "pair_evals_per_s": rate10,
+            "roofline": {"bound": "fp64-valu", "achieved": ach10 / 1e12, "peak": FP64_PEAK / 1e12,
+                         "unit": "TFLOP/s", "frac": ach10 / FP64_PEAK, "traffic": None,
+                         "flops_per_pair": FLOPS_PER_PAIR_ALL,
+                         "valu_issue_frac": valu_issue_frac(rate10, VALU_INSTR_PER_PAIR_ALL,
+                                                            TRANS_PER_PAIR_ALL)}}
+        log(f"ten fields fused: {ms10:.2f} ms/step, {ach10 / FP64_PEAK:.3f} of FP64 peak")
+        if not args.no_cpu and full_size:
+            extra["all_ten_fields"]["parity"] = ten_field_parity(w, prisms, density,
+                                                                 out10.cpu().numpy())
 
     if args.layer:
         extra["layer_path"] = bench_layer_fused(args)

@@ -208,6 +208,16 @@ def spatial_order(easting, northing, bits=16):
     return np.argsort(code, kind="stable")
 
 
+#: Accuracy bar of the mixed-precision fits' predictions (relative to their
+#: largest value, against the FP64 fit) and the conjugate-gradient stop rule
+#: derived from it: the relative update of the data-space prediction in the
+#: last iteration estimates the error left (measured on the 60,000-datum
+#: fit: the error on the 512 x 512 grid is about 1.5 times the last
+#: update), so stopping at a tenth of the bar keeps a margin of ~6.
+MIXED_PREDICTION_BAR = 1e-6
+MIXED_RTOL = MIXED_PREDICTION_BAR / 10
+
+
 def fit_device(coords, data, weights, points, damping, precision="float64", max_iter=None,
                rtol=None, order_xy=None):
     """Coefficients of the scaled, damped, weighted least-squares fit on the
@@ -230,7 +240,7 @@ def fit_device(coords, data, weights, points, damping, precision="float64", max_
     if max_iter is None:
         max_iter = 10 if mode == _lib.HMX_FIT_FP64 else 300
     if rtol is None:
-        rtol = 1e-12 if mode == _lib.HMX_FIT_FP64 else 1e-9
+        rtol = 1e-12 if mode == _lib.HMX_FIT_FP64 else MIXED_RTOL
     if split:
         # The normal equations sum over the data, so their order is free.  In
         # spatial order each group of 1024 data the split normal matrix
@@ -256,7 +266,7 @@ def fit_device(coords, data, weights, points, damping, precision="float64", max_
     check(lib().hmx_eqs_fit_workspace_bytes(n, m, mode, ctypes.byref(nbytes)))
     ws = torch.empty(nbytes.value, dtype=torch.uint8, device=c[0].device)
     coefs = torch.empty(m, dtype=torch.float64, device=c[0].device)
-    info = (ctypes.c_double * 8)()
+    info = (ctypes.c_double * len(FIT_INFO_KEYS))()
     check(lib().hmx_eqs_fit(*(ptr(x) for x in c), n, ptr(d), ptr(w), *(ptr(x) for x in p), m,
                             float(0.0 if damping is None else damping), mode, int(max_iter),
                             float(rtol), ptr(coefs), info, ptr(ws), nbytes.value,
@@ -268,9 +278,11 @@ def fit_device(coords, data, weights, points, damping, precision="float64", max_
 #: gradient iterations, their last relative update, the diagonal shift and
 #: factorisation retries (fits larger than the SVD route), the largest
 #: diagonal entry of the normal matrix, whether the SVD route solved the fit
-#: (1.0) and its Jacobi sweeps and kept singular values.
+#: (1.0), its Jacobi sweeps and kept singular values, and the seconds spent
+#: forming the normal matrix, factorising it, and solving and refining.
 FIT_INFO_KEYS = ("iterations", "last_relative_update", "shift", "retries", "max_diagonal",
-                 "svd", "svd_sweeps", "svd_rank")
+                 "svd", "svd_sweeps", "svd_rank", "normal_matrix_s", "factorisation_s",
+                 "solve_s")
 
 
 # ----------------------------------------------------------- the estimator

@@ -29,6 +29,7 @@ import torch
 from . import _lib
 from ._eqs import (
     FIT_INFO_KEYS,
+    MIXED_RTOL,
     EquivalentSources,
     cast_fit_input,
     check_fit_input,
@@ -171,7 +172,7 @@ class EquivalentSourcesGB(EquivalentSources):
         max_ws = int(max((x.size for x in source_windows), default=1))
         mode = _MODES[self.fit_precision]
         max_iter = 10 if mode == _lib.HMX_FIT_FP64 else 300
-        rtol = 1e-12 if mode == _lib.HMX_FIT_FP64 else 1e-9
+        rtol = 1e-12 if mode == _lib.HMX_FIT_FP64 else MIXED_RTOL
         if self.damping is None:  # LinearRegression's minimum-norm least squares
             mode |= _lib.HMX_FIT_NO_DAMPING
         nbytes = ctypes.c_size_t(0)
@@ -180,7 +181,7 @@ class EquivalentSourcesGB(EquivalentSources):
         coefs = torch.empty(m, dtype=torch.float64, device=dev)
         residue = torch.empty(n, dtype=torch.float64, device=dev)
         rmse = torch.empty(n_windows + 1, dtype=torch.float64, device=dev)
-        info = (ctypes.c_double * 8)()
+        info = (ctypes.c_double * len(FIT_INFO_KEYS))()
         i64p = ctypes.POINTER(ctypes.c_int64)
         check(lib().hmx_eqs_gb_fit(
             *(ptr(x) for x in c), n, ptr(d), ptr(w), *(ptr(x) for x in p), m, ptr(didx),

@@ -610,162 +610,180 @@ __global__ __launch_bounds__(1024) void potrf_diag_kernel(double *__restrict__ D
 }
 
 // ------------------------------------------------------------------ solves
-// Diagonal-block step of the blocked substitutions: out_b = M v_b for the
-// nb x nb triangular inverse M of one diagonal block, with
-//   forward  (U^T y = r):  out[k] = sum_{l <= k} Dinv[l][k] v[l]  (M = Dinv^T)
-//   backward (U x = y):    out[t] = sum_{l >= t} DinvT[l][t] v[l]  (M = Dinv)
-// (Dinv = U_bb^-1 and DinvT its transpose, both row-major kNB x kNB).  One
-// workgroup of 1024 threads: 8 groups of 16 terms per output, consecutive
-// outputs on consecutive lanes (coalesced), partial sums added in order.
-__global__ __launch_bounds__(1024) void trsv_diag_kernel(const double *__restrict__ Minv, int nb,
-                                                        bool lower_terms,
-                                                        const double *__restrict__ v,
-                                                        double *__restrict__ out) {
-    __shared__ double vb[kNB];
-    __shared__ double part[8][kNB];
-    const int t = threadIdx.x & (kNB - 1), g = threadIdx.x >> 7;
-    if (threadIdx.x < nb) vb[threadIdx.x] = v[threadIdx.x];
-    __syncthreads();
-    double acc = 0.0;
-    if (t < nb) {
-#pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            const int l = 16 * g + q;
-            const bool in = l < nb && (lower_terms ? l <= t : l >= t);
-            acc = in ? fma(Minv[l * kNB + t], vb[l], acc) : acc;
+// One-launch blocked substitutions.  Workgroup w of the forward solve
+// (U^T y = r) owns block b = w of y; workgroup w of the backward solve
+// (U x = y) owns block b = nb - 1 - w of x.  A block needs the solved
+// blocks before it (forward) or after it (backward), which belong to
+// workgroups of lower index: each wavefront waits for a block's flag
+// (flags[k] == epoch, polled with agent-scope loads in every lane), reads the
+// block's solution with agent-scope loads, and accumulates its product with
+// the block of U it prefetched into registers before waiting.  The
+// dependencies are visited oldest first, so the wait that matters (the
+// block just before) comes last.  The block's own triangle goes through its
+// stored inverse, then the solution is written and the flag released.
+// Workgroups of a launch are dispatched in index order on each XCD, so the
+// lowest unfinished workgroup always has its dependencies done and the
+// solve makes progress; every wait is still bounded (kTrsvSpinLimit polls)
+// and a timed-out wait sets *err and goes on, so every wavefront ends.
+constexpr int kTrsvSpinLimit = 1 << 22;
+
+// Every lane polls (one request per wavefront: same address) with
+// agent-scope loads, which see the device-wide value without invalidating
+// the caches; all lanes read the same value and leave the loop together.
+// The workgroup-scope acquire fence after the loop keeps the block's loads
+// after it; those are agent-scope loads too (load_agent), so they see what
+// the writer released before its flag.
+__device__ __forceinline__ void wait_flag(const int *flag, int epoch, int *err) {
+    int spins = 0;
+    while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+        if (++spins > kTrsvSpinLimit) {
+            if ((threadIdx.x & 63) == 0) atomicOr(err, 1);
+            break;
         }
+        __builtin_amdgcn_s_sleep(1);
     }
-    part[g][t] = acc;
-    __syncthreads();
-    if (threadIdx.x < nb) {
-        double sum = 0.0;
-#pragma unroll
-        for (int q = 0; q < 8; ++q) sum += part[q][threadIdx.x];
-        out[threadIdx.x] = sum;
-    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
-// Forward substitution U^T y = r, one block step: with y_b (rows
-// [p, p + kNB)) solved, r[c] -= sum_k U[p + k][c] y_b[k] for c in
-// [p + kNB, m), 128 columns per workgroup of 1024 threads (8 groups of 16
-// rows k, consecutive columns on consecutive lanes: coalesced; each lane's
-// 16 loads issued together; partial sums added in order).  Workgroup 0,
-// whose columns are the next block [p + kNB, p + kNB + nb2), then solves it
-// as trsv_diag_kernel does: y_next[t] = sum_{l <= t} Minv_next[l][t] r_next[l].
-__global__ __launch_bounds__(1024) void trsv_fwd_step_kernel(const double *__restrict__ U,
-                                                             int64_t ld, int64_t m, int64_t p,
-                                                             const double *__restrict__ Minv_next,
-                                                             int nb2, double *__restrict__ y,
-                                                             double *__restrict__ r) {
-    __shared__ double yb[kNB];
-    __shared__ double part[8][kNB];
-    __shared__ double rnext[kNB];
-    const int t = threadIdx.x & (kNB - 1), g = threadIdx.x >> 7;
-    if (threadIdx.x < kNB) yb[threadIdx.x] = y[p + threadIdx.x];
-    __syncthreads();
-    const int64_t c = p + kNB + (int64_t)blockIdx.x * kNB + t;
-    double v = 0.0;
-    if (c < m) {
-        const double *col = U + (p + 16 * g) * ld + c;
-        double u[16];
-#pragma unroll
-        for (int k = 0; k < 16; ++k) u[k] = col[k * ld];
-#pragma unroll
-        for (int k = 0; k < 16; ++k) v = fma(u[k], yb[16 * g + k], v);
-    }
-    part[g][t] = v;
-    __syncthreads();
-    double rn = 0.0;
-    if (g == 0 && c < m) {
-        double sum = 0.0;
-#pragma unroll
-        for (int q = 0; q < 8; ++q) sum += part[q][t];
-        rn = r[c] - sum;
-        r[c] = rn;
-    }
-    if (blockIdx.x != 0) return;  // uniform in the workgroup
-    if (g == 0) rnext[t] = rn;
-    __syncthreads();
-    double acc = 0.0;
-    if (t < nb2) {
-        double mv[16];
-#pragma unroll
-        for (int q = 0; q < 16; ++q) mv[q] = Minv_next[(16 * g + q) * kNB + t];
-#pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            const int l = 16 * g + q;
-            acc = (l < nb2 && l <= t) ? fma(mv[q], rnext[l], acc) : acc;
-        }
-    }
-    part[g][t] = acc;
-    __syncthreads();
-    if (g == 0 && t < nb2) {
-        double sum = 0.0;
-#pragma unroll
-        for (int q = 0; q < 8; ++q) sum += part[q][t];
-        y[p + kNB + t] = sum;
-    }
+__device__ __forceinline__ double load_agent(const double *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Backward substitution U x = y, one block step: with x_b (rows [p, p + nb))
-// solved, y[i] -= sum_c U[i][p + c] x_b[c] for i in [0, p), p a multiple of
-// kNB.  Workgroup w (1024 threads) takes rows [p - kNB (w + 1), p - kNB w),
-// each of its 16 wavefronts 8 rows: the 64 lanes read a row's nb values
-// coalesced (the 8 rows' loads issued together) and a butterfly sums them.
-// Workgroup 0, whose rows are the next block [p - kNB, p), then solves it
-// as trsv_diag_kernel does: x_next[t] = sum_{l >= t} Minv_next[l][t] y_next[l].
-__global__ __launch_bounds__(1024) void trsv_bwd_step_kernel(const double *__restrict__ U,
-                                                             int64_t ld, int64_t p, int nb,
-                                                             const double *__restrict__ Minv_next,
-                                                             double *__restrict__ x,
-                                                             double *__restrict__ y) {
-    __shared__ double ynext[kNB];
+// Forward: y_b = Dinv_b^T (r_b - sum_{k < b} U_kb^T y_k), U_kb = rows of
+// block k, columns of block b.  Thread (g, t) = (tid / 128, tid % 128)
+// takes column t of the block and rows 16 g .. 16 g + 15 of every block k.
+__global__ __launch_bounds__(1024) void trsv_fwd_sync_kernel(
+    const double *__restrict__ U, int64_t ld, int64_t m, const double *__restrict__ Dinv,
+    const double *__restrict__ r, double *__restrict__ y, int *__restrict__ flags, int epoch,
+    int *__restrict__ err) {
     __shared__ double part[8][kNB];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const double x0 = lane < nb ? x[p + lane] : 0.0;
-    const double x1 = lane + 64 < nb ? x[p + lane + 64] : 0.0;
-    const int64_t rbase = p - (int64_t)kNB * (blockIdx.x + 1) + 8 * w;  // >= 0
-    double v[8];
+    __shared__ double z[kNB];
+    const int b = blockIdx.x;
+    const int t = threadIdx.x & (kNB - 1), g = threadIdx.x >> 7, lane = threadIdx.x & 63;
+    const int64_t c0 = (int64_t)b * kNB, c = c0 + t;
+    const bool cv = c < m;
+    double mv[16];  // Dinv_b[16 g + q][t]
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        const double *row = U + (rbase + k) * ld + p;
-        double a = lane < nb ? row[lane] * x0 : 0.0;
-        if (lane + 64 < nb) a = fma(row[lane + 64], x1, a);
-        v[k] = a;
-    }
+    for (int q = 0; q < 16; ++q) mv[q] = Dinv[(int64_t)b * kNB * kNB + (16 * g + q) * kNB + t];
+    double u[16];
+    auto load_u = [&](int k) {
+        const double *col = U + ((int64_t)k * kNB + 16 * g) * ld + c;
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1)
-#pragma unroll
-        for (int k = 0; k < 8; ++k) v[k] += __shfl_xor(v[k], off, 64);
-    if (lane < 8) {
-        double d = v[0];
-#pragma unroll
-        for (int k = 1; k < 8; ++k) d = lane == k ? v[k] : d;
-        const int64_t i = rbase + lane;
-        const double yn = y[i] - d;
-        y[i] = yn;
-        if (blockIdx.x == 0) ynext[8 * w + lane] = yn;
-    }
-    if (blockIdx.x != 0) return;  // uniform in the workgroup
-    __syncthreads();
-    const int t = threadIdx.x & (kNB - 1), g = threadIdx.x >> 7;
-    double mv[16];
-#pragma unroll
-    for (int q = 0; q < 16; ++q) mv[q] = Minv_next[(16 * g + q) * kNB + t];
+        for (int q = 0; q < 16; ++q) u[q] = cv ? col[q * ld] : 0.0;
+    };
     double acc = 0.0;
+    if (b > 0) load_u(0);
+    for (int k = 0; k < b; ++k) {
+        wait_flag(flags + k, epoch, err);
+        const double yv = lane < 16 ? load_agent(y + (int64_t)k * kNB + 16 * g + lane) : 0.0;
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-        const int l = 16 * g + q;
-        acc = l >= t ? fma(mv[q], ynext[l], acc) : acc;
+        for (int q = 0; q < 16; ++q) acc = fma(u[q], __shfl(yv, q, 64), acc);
+        if (k + 1 < b) load_u(k + 1);
     }
     part[g][t] = acc;
     __syncthreads();
     if (g == 0) {
-        double sum = 0.0;
+        double s = 0.0;
 #pragma unroll
-        for (int q = 0; q < 8; ++q) sum += part[q][t];
-        x[p - kNB + t] = sum;
+        for (int q = 0; q < 8; ++q) s += part[q][t];
+        z[t] = cv ? r[c] - s : 0.0;
     }
+    __syncthreads();
+    double d = 0.0;  // (Dinv_b^T z)[t] = sum_{l <= t} Dinv_b[l][t] z[l]
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int l = 16 * g + q;
+        d = l <= t ? fma(mv[q], z[l], d) : d;
+    }
+    part[g][t] = d;
+    __syncthreads();
+    if (g == 0 && cv) {
+        double s = 0.0;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) s += part[q][t];
+        y[c] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+        __hip_atomic_store(flags + b, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Backward: x_b = Dinv_b (y_b - sum_{k > b} U_bk x_k).  Wavefront v of the
+// workgroup takes rows 8 v .. 8 v + 7 of the block, lane l columns l and
+// l + 64 of every block k (partial sums per lane, added across the lanes
+// once at the end).
+__global__ __launch_bounds__(1024) void trsv_bwd_sync_kernel(
+    const double *__restrict__ U, int64_t ld, int64_t m, const double *__restrict__ DinvT,
+    const double *__restrict__ y, double *__restrict__ x, int *__restrict__ flags, int epoch,
+    int *__restrict__ err) {
+    __shared__ double part[8][kNB];
+    __shared__ double z[kNB];
+    const int nb = (int)((m + kNB - 1) / kNB);
+    const int b = nb - 1 - (int)blockIdx.x;
+    const int lane = threadIdx.x & 63, v = threadIdx.x >> 6;
+    const int t = threadIdx.x & (kNB - 1), g = threadIdx.x >> 7;
+    const int64_t r0 = (int64_t)b * kNB;
+    double mv[16];  // DinvT_b[16 g + q][t] = Dinv_b[t][16 g + q]
+#pragma unroll
+    for (int q = 0; q < 16; ++q) mv[q] = DinvT[(int64_t)b * kNB * kNB + (16 * g + q) * kNB + t];
+    double u[16];
+    auto load_u = [&](int k) {
+        const int64_t cb = (int64_t)k * kNB;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int64_t row = r0 + 8 * v + i;
+            const double *p = U + (row < m ? row : m - 1) * ld + cb;
+            u[2 * i] = (row < m && cb + lane < m) ? p[lane] : 0.0;
+            u[2 * i + 1] = (row < m && cb + lane + 64 < m) ? p[lane + 64] : 0.0;
+        }
+    };
+    double acc[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) acc[i] = 0.0;
+    if (b + 1 < nb) load_u(nb - 1);
+    for (int k = nb - 1; k > b; --k) {
+        wait_flag(flags + k, epoch, err);
+        const int64_t cb = (int64_t)k * kNB;
+        const double x0 = cb + lane < m ? load_agent(x + cb + lane) : 0.0;
+        const double x1 = cb + lane + 64 < m ? load_agent(x + cb + lane + 64) : 0.0;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) acc[i] = fma(u[2 * i + 1], x1, fma(u[2 * i], x0, acc[i]));
+        if (k - 1 > b) load_u(k - 1);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+        for (int i = 0; i < 8; ++i) acc[i] += __shfl_xor(acc[i], off, 64);
+    if (lane < 8) {
+        double s = acc[0];
+#pragma unroll
+        for (int i = 1; i < 8; ++i) s = lane == i ? acc[i] : s;
+        const int64_t row = r0 + 8 * v + lane;
+        z[8 * v + lane] = row < m ? y[row] - s : 0.0;
+    }
+    __syncthreads();
+    double d = 0.0;  // (Dinv_b z)[t] = sum_{l >= t} Dinv_b[t][l] z[l]
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int l = 16 * g + q;
+        d = l >= t ? fma(mv[q], z[l], d) : d;
+    }
+    part[g][t] = d;
+    __syncthreads();
+    if (g == 0 && r0 + t < m) {
+        double s = 0.0;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) s += part[q][t];
+        x[r0 + t] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+        __hip_atomic_store(flags + b, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ void fill_int_kernel(int *__restrict__ p, int64_t n, int v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
 }
 
 // ------------------------------------------------------ matrix-free A p
@@ -1308,6 +1326,11 @@ inline unsigned blocks_for(int64_t n, int per = 256) {
     return (unsigned)(b < 1 ? 1 : b);
 }
 
+// Bytes of the substitutions' dependency flags (2 per block) and error word.
+inline size_t solve_flag_bytes(int64_t m) {
+    return (size_t)(2 * ((m + kNB - 1) / kNB) + 1) * sizeof(int);
+}
+
 // Whether a fit takes the SVD route when its Cholesky route cannot match
 // the reference's solver (the route is sized for these fits only).
 inline bool svd_route_fits(int64_t n_data, int64_t n_src) {
@@ -1338,6 +1361,28 @@ inline size_t fit_x_bytes(int64_t n_data, int64_t n_src, int mode) {
     return x_bytes;
 }
 
+// Stream events at the phase boundaries of a fit (destroyed on every
+// return path); elapsed times in seconds.
+struct PhaseEvents {
+    hipEvent_t e[4] = {};
+    PhaseEvents() {
+        for (auto &x : e)
+            if (hipEventCreate(&x) != hipSuccess) x = nullptr;
+    }
+    ~PhaseEvents() {
+        for (auto &x : e)
+            if (x) hipEventDestroy(x);
+    }
+    void record(int k, hipStream_t s) {
+        if (e[k]) hipEventRecord(e[k], s);
+    }
+    double seconds(int a, int b) {
+        float ms = 0.0f;
+        if (!e[a] || !e[b] || hipEventElapsedTime(&ms, e[a], e[b]) != hipSuccess) return 0.0;
+        return 1e-3 * ms;
+    }
+};
+
 // Pivot ratio below which an unregularised fit (damping=None) is solved by
 // the SVD route: min U_ii^2 / max U_ii^2 >= 1 / cond(X)^2, so above it the
 // scaled jacobian X has cond(X) < 1e6 and lstsq's cut-off (max(n, m)
@@ -1355,6 +1400,10 @@ struct Fit {
     double *scale, *A, *Dinv, *DinvT, *tmp, *part;
     double host_part[256];
     int *dinfo;
+    // dependency flags of the two substitution launches (2 per block), the
+    // error word of their waits, and the epoch of the last solve
+    int *flags = nullptr, *terr = nullptr;
+    int epoch = 0;
     double *split = nullptr;  // partial sums of the split reductions
 
     // out = (1 / s) J^T W t + alpha p (W = diag(w_), w_ and p may be NULL).
@@ -1462,37 +1511,37 @@ struct Fit {
         return HMX_OK;
     }
 
-    // x = A^-1 r with the factor; r is overwritten (work), y is scratch.
-    // Per kNB block: the diagonal-block product writes the block's solution,
-    // then the update kernel applies it to the rest of the right-hand side.
-    // Blocked triangular solves: U^T y = r (forward), then U x = y
-    // (backward), with the diagonal blocks' inverses.  The first block of
-    // each direction is solved by trsv_diag_kernel; every later one by the
-    // update kernel of the block before it (its workgroup 0).  r and y are
-    // overwritten.
+    // Flags and error word at zero, before the first solve.
+    int init_solves() {
+        const int64_t n = 2 * ((m + kNB - 1) / kNB) + 1;
+        hipLaunchKernelGGL(fill_int_kernel, dim3(blocks_for(n)), dim3(256), 0, s, flags, n, 0);
+        HMX_LAUNCHED("fill_int_kernel");
+        terr = flags + n - 1;
+        epoch = 0;
+        return HMX_OK;
+    }
+
+    // x = A^-1 r with the factor: U^T y = r (forward), then U x = y
+    // (backward), one launch each (y is scratch; r is not modified).
     int solve(double *r, double *y, double *x) {
-        const int nb0 = (int)(m < kNB ? m : kNB);
-        hipLaunchKernelGGL(trsv_diag_kernel, dim3(1), dim3(1024), 0, s, Dinv, nb0, true, r, y);
-        HMX_LAUNCHED("trsv_diag_kernel");
-        for (int64_t p = 0; p + kNB < m; p += kNB) {  // block p is full: another follows
-            const int64_t rest = m - p - kNB;
-            const int nb2 = (int)(rest < kNB ? rest : kNB);
-            hipLaunchKernelGGL(trsv_fwd_step_kernel, dim3((unsigned)((rest + kNB - 1) / kNB)),
-                               dim3(1024), 0, s, A, m, m, p, Dinv + (p / kNB + 1) * kNB * kNB,
-                               nb2, y, r);
-            HMX_LAUNCHED("trsv_fwd_step_kernel");
-        }
-        const int64_t last = ((m - 1) / kNB) * kNB;
-        hipLaunchKernelGGL(trsv_diag_kernel, dim3(1), dim3(1024), 0, s,
-                           DinvT + (last / kNB) * kNB * kNB, (int)(m - last), false, y + last,
-                           x + last);
-        HMX_LAUNCHED("trsv_diag_kernel");
-        for (int64_t p = last; p > 0; p -= kNB) {
-            const int nb = (int)(m - p < kNB ? m - p : kNB);
-            hipLaunchKernelGGL(trsv_bwd_step_kernel, dim3((unsigned)(p / kNB)), dim3(1024), 0, s,
-                               A, m, p, nb, DinvT + (p / kNB - 1) * kNB * kNB, x, y);
-            HMX_LAUNCHED("trsv_bwd_step_kernel");
-        }
+        const int nb = (int)((m + kNB - 1) / kNB);
+        ++epoch;
+        hipLaunchKernelGGL(trsv_fwd_sync_kernel, dim3(nb), dim3(1024), 0, s, A, m, m, Dinv, r, y,
+                           flags, epoch, terr);
+        HMX_LAUNCHED("trsv_fwd_sync_kernel");
+        hipLaunchKernelGGL(trsv_bwd_sync_kernel, dim3(nb), dim3(1024), 0, s, A, m, m, DinvT, y, x,
+                           flags + nb, epoch, terr);
+        HMX_LAUNCHED("trsv_bwd_sync_kernel");
+        return HMX_OK;
+    }
+
+    // Fails if a wait of the substitutions timed out (synchronises).
+    int check_solves() {
+        if (!terr) return HMX_OK;  // no solve ran
+        int e = 0;
+        HMX_HIP(hipMemcpyAsync(&e, terr, sizeof(int), hipMemcpyDeviceToHost, s));
+        HMX_HIP(hipStreamSynchronize(s));
+        if (e) return fail(HMX_ERR_NUMERIC, "triangular solve: a dependency wait timed out");
         return HMX_OK;
     }
 };
@@ -1563,7 +1612,8 @@ int hmx_cholesky_workspace_bytes(int64_t m, size_t *bytes) {
     if (!bytes) return fail(HMX_ERR_ARG, "bytes is NULL");
     if (m <= 0) return fail(HMX_ERR_ARG, "empty matrix");
     const size_t nblk = (size_t)((m + kNB - 1) / kNB);
-    *bytes = 2 * nblk * kNB * kNB * sizeof(double) + (2 * (size_t)m + 256) * sizeof(double) + 256;
+    *bytes = 2 * nblk * kNB * kNB * sizeof(double) + (2 * (size_t)m + 256) * sizeof(double) + 256 +
+             solve_flag_bytes(m);
     return HMX_OK;
 }
 
@@ -1588,15 +1638,17 @@ int hmx_cholesky_solve(double *A, int64_t m, double *x, int *info, void *workspa
     double *wk2 = wk + m;
     f.part = wk2 + m;
     f.dinfo = reinterpret_cast<int *>(f.part + 256);
+    f.flags = f.dinfo + 64;
     st = f.cholesky(info);
     if (st) return st;
     if (*info) return HMX_OK;
+    st = f.init_solves();
+    if (st) return st;
     hipLaunchKernelGGL(copy_kernel, dim3(blocks_for(m)), dim3(256), 0, f.s, m, x, wk);
     HMX_LAUNCHED("copy_kernel");
     st = f.solve(wk, wk2, x);
     if (st) return st;
-    HMX_HIP(hipStreamSynchronize(f.s));
-    return HMX_OK;
+    return f.check_solves();
 }
 
 int hmx_eqs_fit_workspace_bytes(int64_t n_data, int64_t n_src, int precision_mode,
@@ -1615,7 +1667,8 @@ int hmx_eqs_fit_workspace_bytes(int64_t n_data, int64_t n_src, int precision_mod
              + fit_x_bytes(n_data, n_src, mode)        // the scaled jacobian (or its slices)
              + 2 * nblk * kNB * kNB * sizeof(double)  // inverses of diagonal blocks
              + (10 * m + n + 256) * sizeof(double)  // 10 m-vectors, n-vector, partials
-             + split_doubles(m > n ? m : n) * sizeof(double) + 256;  // split sums, info
+             + split_doubles(m > n ? m : n) * sizeof(double) + 256  // split sums, info
+             + solve_flag_bytes(n_src);                             // substitution flags
     return HMX_OK;
 }
 
@@ -1691,7 +1744,12 @@ int hmx_eqs_fit(const double *easting, const double *northing, const double *upw
     f.part = f.tmp + nd;
     f.split = f.part + 256;
     f.dinfo = reinterpret_cast<int *>(f.split + split_doubles(m > nd ? m : nd));
+    f.flags = f.dinfo + 64;  // after the 256 bytes of info
     const unsigned gm = blocks_for(m), gy = (unsigned)(nd < 8192 ? nd : 8192);
+
+    // phases: 0 start, 1 factorisation start, 2 factorisation end, 3 end
+    PhaseEvents ev;
+    ev.record(0, s);
 
     // 1. column scales, right-hand side b = S J^T W d (FP64).
     st = f.col_stats();
@@ -1796,6 +1854,7 @@ int hmx_eqs_fit(const double *easting, const double *northing, const double *upw
                                      damping + tau, s);
             if (st) return st;
         }
+        if (attempts == 0) ev.record(1, s);
         st = f.cholesky(&pivot);
         if (st) return st;
         if (pivot == 0) break;
@@ -1819,6 +1878,8 @@ int hmx_eqs_fit(const double *easting, const double *northing, const double *upw
         tau = tau > 0.0 ? 10.0 * tau : tau0;
     }
 
+    ev.record(2, s);
+
     // An unregularised fit whose factor has a small pivot ratio may have
     // singular values below lstsq's cut-off: SVD route.
     if (lstsq && svd_ok && !use_svd) {
@@ -1832,6 +1893,8 @@ int hmx_eqs_fit(const double *easting, const double *northing, const double *upw
 
     // 3. x = (factor)^-1 b.
     if (!use_svd) {
+        st = f.init_solves();
+        if (st) return st;
         hipLaunchKernelGGL(copy_kernel, dim3(gm), dim3(256), 0, s, m, b, wk);
         HMX_LAUNCHED("copy_kernel");
         st = f.solve(wk, wk2, x);
@@ -1942,7 +2005,10 @@ int hmx_eqs_fit(const double *easting, const double *northing, const double *upw
     // 5. c = S c'.
     hipLaunchKernelGGL(div_kernel, dim3(gm), dim3(256), 0, s, m, x, f.scale, coefs);
     HMX_LAUNCHED("div_kernel");
+    ev.record(3, s);
     HMX_HIP(hipStreamSynchronize(s));
+    st = f.check_solves();
+    if (st) return st;
     info[0] = (double)iters;
     info[1] = last_rel;
     info[2] = tau;
@@ -1951,6 +2017,9 @@ int hmx_eqs_fit(const double *easting, const double *northing, const double *upw
     info[5] = use_svd ? 1.0 : 0.0;
     info[6] = (double)sweeps;
     info[7] = (double)rank;
+    info[8] = ev.seconds(0, 1);  // column statistics, jacobian and normal matrix
+    info[9] = ev.seconds(1, 2);  // factorisation (with any retries)
+    info[10] = ev.seconds(2, 3);  // solves, refinement or the SVD route
     return HMX_OK;
 }
 

@@ -248,6 +248,7 @@ int hmx_eqs_gb_fit(const double *easting, const double *northing, const double *
 
     double total_iter = 0.0, max_shift = 0.0, total_retries = 0.0;
     double last_update = 0.0, max_diag = 0.0, svd_windows = 0.0, svd_sweeps = 0.0, svd_rank = 0.0;
+    double phase[3] = {0.0, 0.0, 0.0};
     for (int64_t w = 0; w < n_windows; ++w) {
         const int64_t d0 = data_offset[w], nd = data_offset[w + 1] - d0;
         const int64_t s0 = source_offset[w], ns = source_offset[w + 1] - s0;
@@ -258,7 +259,7 @@ int hmx_eqs_gb_fit(const double *easting, const double *northing, const double *
                            wd[0], wd[1],
                            wd[2], wd[3], wd[4], wsrc[0], wsrc[1], wsrc[2]);
         HMX_LAUNCHED("gb_gather_kernel");
-        double winfo[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        double winfo[11] = {};
         st = hmx_eqs_fit(wd[0], wd[1], wd[2], nd, wd[3], weights ? wd[4] : nullptr, wsrc[0],
                          wsrc[1], wsrc[2], ns, damping, precision_mode, max_iter, rtol, wsrc[3],
                          winfo, fit_ws, L.fit_bytes, stream);
@@ -271,6 +272,7 @@ int hmx_eqs_gb_fit(const double *easting, const double *northing, const double *
         svd_windows += winfo[5];
         svd_sweeps += winfo[6] < 0 ? -winfo[6] : winfo[6];
         svd_rank += winfo[7];
+        for (int k = 0; k < 3; ++k) phase[k] += winfo[8 + k];
         st = hmx_eqs_predict(easting, northing, upward, n_data, wsrc[0], wsrc[1], wsrc[2], wsrc[3],
                              ns, HMX_F64, pred, pred_ws, L.pred_ws_bytes, stream);
         if (st) return st;
@@ -292,6 +294,7 @@ int hmx_eqs_gb_fit(const double *easting, const double *northing, const double *
     info[5] = svd_windows;
     info[6] = svd_sweeps;
     info[7] = svd_rank;
+    for (int k = 0; k < 3; ++k) info[8 + k] = phase[k];
     HMX_HIP(hipStreamSynchronize(s));
     return HMX_OK;
 }

@@ -354,10 +354,12 @@ int hmx_eqs_fit_workspace_bytes(int64_t n_data, int64_t n_src, int precision_mod
  *   max_iter, rtol: CG limit and stop rule (relative change of the
  *     data-space prediction per iteration);
  *   coefs: (n_src) device output;
- *   info: HOST array of 8 doubles: [CG iterations, last relative update,
+ *   info: HOST array of 11 doubles: [CG iterations, last relative update,
  *     shift tau, factorisation retries, max diagonal of the normal matrix,
  *     1 if the SVD route solved the fit, its Jacobi sweeps (negative if
- *     the sweep limit stopped them), its kept singular values].
+ *     the sweep limit stopped them), its kept singular values, seconds of
+ *     the normal matrix (column statistics and jacobian included), of the
+ *     factorisation and of the solves and refinement (stream events)].
  * Synchronous: returns after the coefficients are written. */
 int hmx_eqs_fit(const double *easting, const double *northing, const double *upward,
                 int64_t n_data, const double *data, const double *weights,
@@ -415,10 +417,11 @@ int hmx_eqs_gb_fit_workspace_bytes(int64_t n_data, int64_t max_window_data,
  *   rmse: device output of n_windows + 1 values, the RMS of the residue
  *     before the first window and after each (rmse_per_iteration_, :250,
  *     :278);
- *   info: HOST array of 8 doubles: [total CG iterations, last relative
+ *   info: HOST array of 11 doubles: [total CG iterations, last relative
  *     update, largest shift, total factorisation retries, largest diagonal,
  *     windows solved by the SVD route, their total Jacobi sweeps, their
- *     total kept singular values].
+ *     total kept singular values, and the phase seconds of hmx_eqs_fit
+ *     summed over the windows].
  * Indices outside [0, n_data) or [0, n_src) give NaN, never an access out of
  * bounds.  Synchronous. */
 int hmx_eqs_gb_fit(const double *easting, const double *northing, const double *upward,

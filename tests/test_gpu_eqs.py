@@ -295,7 +295,8 @@ def test_mixed_fit_small_and_ragged(hm, precision, n_data, n_src):
     pts = (rng.uniform(0, L, n_src), rng.uniform(0, L, n_src), rng.uniform(-3000, -1000, n_src))
     data = rng.normal(0, 1, n_data)
     c64, _ = fit_device(coords, data, None, pts, 1.0, "float64")
-    cmx, info = fit_device(coords, data, None, pts, 1.0, precision)
+    # refined past the default stop rule, so the coefficients themselves agree
+    cmx, info = fit_device(coords, data, None, pts, 1.0, precision, rtol=1e-11)
     assert np.all(np.isfinite(cmx)), info
     npt.assert_allclose(cmx, c64, rtol=0, atol=1e-7 * np.abs(c64).max())
 
