@@ -77,6 +77,14 @@ def progressbar_context(total, use_progressbar):
 
 
 # -------------------------------------------------------- device-level API
+def on_stream(stream):
+    """Context in which the output and workspace of a device-level call are
+    allocated: the caller's stream when one is given (so torch's caching
+    allocator orders their reuse after the kernel on that stream), else the
+    current stream."""
+    return torch.cuda.stream(stream) if stream is not None else contextlib.nullcontext()
+
+
 def prism_fields_device(easting, northing, upward, prisms, density, fields, stream=None,
                         flags=None):
     """Fields of prisms on observation points, all tensors already in HBM.
@@ -88,11 +96,13 @@ def prism_fields_device(easting, northing, upward, prisms, density, fields, stre
     """
     mask = field_mask(fields)
     n_obs, n_prisms = easting.numel(), prisms.shape[0]
-    out = torch.empty((bin(mask).count("1"), n_obs), dtype=torch.float64, device=easting.device)
-    ws, nbytes = _workspace(n_obs, n_prisms, mask)
-    check(lib().hmx_prism_gravity(ptr(easting), ptr(northing), ptr(upward), n_obs, ptr(prisms),
-                                  ptr(density), n_prisms, mask, ptr(out), ptr(flags), ptr(ws),
-                                  nbytes, stream_handle(stream)))
+    with on_stream(stream):
+        out = torch.empty((bin(mask).count("1"), n_obs), dtype=torch.float64,
+                          device=easting.device)
+        ws, nbytes = _workspace(n_obs, n_prisms, mask)
+        check(lib().hmx_prism_gravity(ptr(easting), ptr(northing), ptr(upward), n_obs,
+                                      ptr(prisms), ptr(density), n_prisms, mask, ptr(out),
+                                      ptr(flags), ptr(ws), nbytes, stream_handle(stream)))
     return out
 
 
@@ -102,13 +112,15 @@ def layer_fields_device(easting, northing, upward, centre_easting, centre_northi
     mask = field_mask(fields)
     n_obs = easting.numel()
     n_e, n_n = centre_easting.numel(), centre_northing.numel()
-    out = torch.empty((bin(mask).count("1"), n_obs), dtype=torch.float64, device=easting.device)
-    ws, nbytes = _workspace(n_obs, n_e * n_n, mask)
-    check(lib().hmx_prism_layer_gravity(ptr(easting), ptr(northing), ptr(upward), n_obs,
-                                        ptr(centre_easting), n_e, ptr(centre_northing), n_n,
-                                        ptr(bottom), ptr(top), ptr(density),
-                                        float(thickness_threshold), mask, ptr(out), ptr(flags),
-                                        ptr(ws), nbytes, stream_handle(stream)))
+    with on_stream(stream):
+        out = torch.empty((bin(mask).count("1"), n_obs), dtype=torch.float64,
+                          device=easting.device)
+        ws, nbytes = _workspace(n_obs, n_e * n_n, mask)
+        check(lib().hmx_prism_layer_gravity(ptr(easting), ptr(northing), ptr(upward), n_obs,
+                                            ptr(centre_easting), n_e, ptr(centre_northing), n_n,
+                                            ptr(bottom), ptr(top), ptr(density),
+                                            float(thickness_threshold), mask, ptr(out),
+                                            ptr(flags), ptr(ws), nbytes, stream_handle(stream)))
     return out
 
 
@@ -116,11 +128,12 @@ def point_fields_device(c0, c1, c2, p0, p1, p2, masses, fields, coordinate_syste
     """Fields of point masses (all tensors in HBM)."""
     mask = field_mask(fields)
     n_obs, n_pts = c0.numel(), p0.numel()
-    out = torch.empty((bin(mask).count("1"), n_obs), dtype=torch.float64, device=c0.device)
-    ws, nbytes = _workspace(n_obs, n_pts, mask)
-    check(lib().hmx_point_gravity(ptr(c0), ptr(c1), ptr(c2), n_obs, ptr(p0), ptr(p1), ptr(p2),
-                                  ptr(masses), n_pts, int(coordinate_system), mask, ptr(out),
-                                  ptr(ws), nbytes, stream_handle(stream)))
+    with on_stream(stream):
+        out = torch.empty((bin(mask).count("1"), n_obs), dtype=torch.float64, device=c0.device)
+        ws, nbytes = _workspace(n_obs, n_pts, mask)
+        check(lib().hmx_point_gravity(ptr(c0), ptr(c1), ptr(c2), n_obs, ptr(p0), ptr(p1),
+                                      ptr(p2), ptr(masses), n_pts, int(coordinate_system), mask,
+                                      ptr(out), ptr(ws), nbytes, stream_handle(stream)))
     return out
 
 

@@ -11,7 +11,8 @@ import ctypes
 import numpy as np
 import torch
 
-from ._forward import _prepare_coordinates, check_prisms, progressbar_context, to_device
+from ._forward import (_prepare_coordinates, check_prisms, on_stream, progressbar_context,
+                       to_device)
 from ._lib import check, lib, ptr, stream_handle
 
 VALID_FIELDS = ("b", "b_e", "b_n", "b_u")
@@ -29,12 +30,14 @@ def magnetic_fields_device(easting, northing, upward, prisms, mag_e, mag_n, mag_
     n_obs, n_prisms = easting.numel(), prisms.shape[0]
     nbytes = ctypes.c_size_t(0)
     check(lib().hmx_prism_magnetic_workspace_bytes(n_obs, n_prisms, mask, ctypes.byref(nbytes)))
-    ws = torch.empty(max(nbytes.value, 8), dtype=torch.uint8, device=easting.device)
     nc = bin(mask).count("1")
-    out = torch.empty((nc, n_obs), dtype=torch.float64, device=easting.device)
-    check(lib().hmx_prism_magnetic(ptr(easting), ptr(northing), ptr(upward), n_obs, ptr(prisms),
-                                   ptr(mag_e), ptr(mag_n), ptr(mag_u), n_prisms, mask, ptr(out),
-                                   ptr(ws), nbytes.value, stream_handle(stream)))
+    with on_stream(stream):
+        ws = torch.empty(max(nbytes.value, 8), dtype=torch.uint8, device=easting.device)
+        out = torch.empty((nc, n_obs), dtype=torch.float64, device=easting.device)
+        check(lib().hmx_prism_magnetic(ptr(easting), ptr(northing), ptr(upward), n_obs,
+                                       ptr(prisms), ptr(mag_e), ptr(mag_n), ptr(mag_u), n_prisms,
+                                       mask, ptr(out), ptr(ws), nbytes.value,
+                                       stream_handle(stream)))
     # Rows come in bit order (b_e, b_n, b_u); reorder to the request.
     order = sorted(components, key=lambda c: COMPONENT_BIT[c])
     if list(order) == list(components):

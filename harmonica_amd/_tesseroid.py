@@ -198,6 +198,30 @@ def _node_densities(tesseroids, density):
     return out
 
 
+#: Chebyshev coefficients per tesseroid of the density's radial profile
+#: (hmx_tesseroid_gravity evaluates them at the radial nodes of the parts
+#: that the radial discretisation creates).
+CHEB_N = 32
+
+
+def _chebyshev_densities(tesseroids, density):
+    """(m, CHEB_N) Chebyshev coefficients c_k of each tesseroid's density
+    over its radial extent: the interpolant sum_k c_k T_k(x) through the
+    density at the CHEB_N Chebyshev points x_j = cos(pi (j + 1/2) / CHEB_N)
+    of [bottom, top] (c_k = (2 / N) sum_j f_j T_k(x_j), c_0 halved).  For
+    the density profiles of the reference's tests (polynomials, exponentials)
+    it matches the function to rounding; the tests state its error."""
+    j = np.arange(CHEB_N)
+    x = np.cos(np.pi * (j + 0.5) / CHEB_N)
+    bottom, top = tesseroids[:, 4], tesseroids[:, 5]
+    radii = 0.5 * (top + bottom)[:, None] + 0.5 * (top - bottom)[:, None] * x[None, :]
+    f = np.array([[density(r) for r in row] for row in radii], dtype=np.float64)
+    T = np.cos(np.outer(np.arange(CHEB_N), np.pi * (j + 0.5) / CHEB_N))  # T_k(x_j)
+    c = (2.0 / CHEB_N) * f @ T.T
+    c[:, 0] *= 0.5
+    return c
+
+
 def tesseroid_gravity(
     coordinates,
     tesseroids,
@@ -218,8 +242,11 @@ def tesseroid_gravity(
     rows (west, east, south, north, bottom, top), ``field`` "potential"
     (J/kg) or "g_z" (mGal, downward).  ``density`` is an array (kg/m^3) or a
     function of the radius; a function runs the reference's density-based
-    discretisation on the host and is supported without
-    ``radial_adaptive_discretization``.  The adaptive discretisation and the
+    discretisation on the host, which evaluates it at the radial quadrature
+    nodes of every piece and, for ``radial_adaptive_discretization``,
+    tabulates it per piece as 32 Chebyshev coefficients that the GPU
+    evaluates at the nodes of the radial parts (to within 1e-13 of the
+    function for smooth profiles).  The adaptive discretisation and the
     quadrature run on the GPU in FP64 (``parallel`` is accepted).
     """
     if field not in ("potential", "g_z"):
@@ -228,17 +255,16 @@ def tesseroid_gravity(
     cast, coords = _prepare_coordinates(coordinates)
     tesseroids = np.atleast_2d(np.asarray(tesseroids, dtype=np.float64))
     if not disable_checks:
+        # the checks move west > east tesseroids to [-180, 180); without
+        # them such tesseroids are evaluated as given, like the reference
         tesseroids = _check_tesseroids(tesseroids)
         check_points_outside_tesseroids(coords, tesseroids)
-    elif (tesseroids[:, 0] > tesseroids[:, 1]).any():
-        tesseroids = _longitude_continuity(tesseroids)
+    cheb = None
     if callable(density):
-        if radial_adaptive_discretization:
-            raise NotImplementedError(
-                "Density functions with radial_adaptive_discretization=True are not supported "
-                "on the GPU: the radial quadrature nodes are only known on the device.")
         tesseroids = density_based_discretization(tesseroids, density)
         rho = _node_densities(tesseroids, density)
+        if radial_adaptive_discretization:
+            cheb = _chebyshev_densities(tesseroids, density)
     else:
         density = np.atleast_1d(np.asarray(density, dtype=np.float64)).ravel()
         if not disable_checks and density.size != tesseroids.shape[0]:
@@ -261,13 +287,15 @@ def tesseroid_gravity(
         n, m = lon.numel(), tesseroids.shape[0]
         T = to_device(np.ascontiguousarray(tesseroids.reshape(-1, 6)))
         D = to_device(np.ascontiguousarray(rho.reshape(-1, 2)))
+        C = None if cheb is None else to_device(np.ascontiguousarray(cheb))
         nbytes = ctypes.c_size_t(0)
         check(lib().hmx_tesseroid_workspace_bytes(n, m, ctypes.byref(nbytes)))
         ws = torch.empty(max(nbytes.value, 8), dtype=torch.uint8, device=lon.device)
         out = torch.empty(n, dtype=torch.float64, device=lon.device)
         ovf = torch.zeros(1, dtype=torch.int32, device=lon.device)
         bit = _lib.FIELD_BIT[field]
-        check(lib().hmx_tesseroid_gravity(ptr(lon), ptr(lat), ptr(rad), n, ptr(T), ptr(D), m, bit,
+        check(lib().hmx_tesseroid_gravity(ptr(lon), ptr(lat), ptr(rad), n, ptr(T), ptr(D), ptr(C),
+                                          m, bit,
                                           int(bool(radial_adaptive_discretization)), ptr(out),
                                           ptr(ovf), ptr(ws), nbytes.value, stream_handle(None)))
         flag = int(ovf.item())

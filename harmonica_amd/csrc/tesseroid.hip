@@ -173,12 +173,35 @@ __device__ __forceinline__ bool tess_push_parts(double (&stk)[kTessStack][6], in
     return true;
 }
 
+// Density of a tesseroid's radial profile from its Chebyshev coefficients
+// c[0..kChebN) over [B, T] (the undivided tesseroid's radial extent):
+// sum_k c_k T_k(x), x = (2 r - (B + T)) / (T - B), by Clenshaw's recurrence.
+constexpr int kChebN = 32;
+__device__ __forceinline__ double cheb_density(const double *__restrict__ c, double r, double B,
+                                               double T) {
+    double x = (2.0 * r - (B + T)) / (T - B);
+    x = fmin(1.0, fmax(-1.0, x));
+    double b1 = 0.0, b2 = 0.0;
+#pragma unroll
+    for (int k = kChebN - 1; k >= 1; --k) {
+        const double b0 = fma(2.0 * x, b1, c[k] - b2);
+        b2 = b1;
+        b1 = b0;
+    }
+    return fma(x, b1, c[0] - b2);
+}
+
 // One (point, tesseroid) pair: adaptive discretisation with the lane's
 // stack, small tesseroids integrated in the order they are produced.  The
 // undivided tesseroid is tested and, if small enough, integrated from its
 // precomputed constants; its parts go through the generic path.
+// With radial discretisation and a variable density (cheb: the tesseroid's
+// Chebyshev coefficients, else NULL), the parts' radial nodes take their
+// densities from the coefficients; otherwise they share the undivided
+// tesseroid's nodes (radially undivided parts) or its constant density.
 template <bool kPot, bool kRadial>
-__device__ double tess_pair(const TessPoint &P, const TessRoot &R, double ratio, int *ovf) {
+__device__ double tess_pair(const TessPoint &P, const TessRoot &R, double ratio, int *ovf,
+                            const double *__restrict__ cheb) {
     {
         // _distance_tesseroid_point from the centre
         const double coslambda = cos(R.lon_c - P.lon_rad);
@@ -219,7 +242,18 @@ __device__ double tess_pair(const TessPoint &P, const TessRoot &R, double ratio,
                     return 0.0;
                 }
                 ++n_small;
-                result += tess_glq<kPot>(P, t, R.rho);
+                if (kRadial && cheb) {
+                    double rho_p[2];
+                    const double node[2] = {-kGlqNode, kGlqNode};
+#pragma unroll
+                    for (int k = 0; k < 2; ++k)
+                        rho_p[k] = cheb_density(
+                            cheb, 0.5 * (t[5] - t[4]) * node[k] + 0.5 * (t[5] + t[4]), R.t[4],
+                            R.t[5]);
+                    result += tess_glq<kPot>(P, t, rho_p);
+                } else {
+                    result += tess_glq<kPot>(P, t, R.rho);
+                }
             }
         }
         return result;
@@ -262,8 +296,8 @@ template <bool kPot, bool kRadial>
 __global__ __launch_bounds__(kObsPerBlock) void tesseroid_kernel(
     const double *__restrict__ lon, const double *__restrict__ lat,
     const double *__restrict__ rad, int n_obs, const TessRoot *__restrict__ roots,
-    int64_t n_src, int64_t chunk_len, double ratio, double *__restrict__ partial,
-    int *__restrict__ overflow) {
+    int64_t n_src, int64_t chunk_len, double ratio, const double *__restrict__ cheb,
+    double *__restrict__ partial, int *__restrict__ overflow) {
     __shared__ TessRoot tile[kTile];
     const int i = blockIdx.x * kObsPerBlock + threadIdx.x;
     const bool valid = i < n_obs;
@@ -286,7 +320,9 @@ __global__ __launch_bounds__(kObsPerBlock) void tesseroid_kernel(
         double *dst = reinterpret_cast<double *>(tile);
         for (int q = threadIdx.x; q < cnt * kRootDoubles; q += blockDim.x) dst[q] = src[q];
         __syncthreads();
-        for (int p = 0; p < cnt && !ovf; ++p) acc += tess_pair<kPot, kRadial>(P, tile[p], ratio, &ovf);
+        for (int p = 0; p < cnt && !ovf; ++p)
+            acc += tess_pair<kPot, kRadial>(P, tile[p], ratio, &ovf,
+                                            cheb ? cheb + (t0 + p) * kChebN : nullptr);
     }
     if (valid) {
         partial[(int64_t)blockIdx.y * n_obs + i] = acc;
@@ -336,9 +372,9 @@ int hmx_tesseroid_workspace_bytes(int64_t n_obs, int64_t n_tesseroids, size_t *b
 
 int hmx_tesseroid_gravity(const double *longitude, const double *latitude, const double *radius,
                           int64_t n_obs, const double *tesseroids, const double *density_nodes,
-                          int64_t n_tesseroids, uint32_t field, int radial_discretization,
-                          double *out, int *overflow, void *workspace, size_t workspace_bytes,
-                          void *stream) {
+                          const double *density_cheb, int64_t n_tesseroids, uint32_t field,
+                          int radial_discretization, double *out, int *overflow, void *workspace,
+                          size_t workspace_bytes, void *stream) {
     using namespace hmx;
     clear_error();
     if (n_obs < 0 || n_tesseroids < 0) return fail(HMX_ERR_ARG, "negative size");
@@ -377,7 +413,7 @@ int hmx_tesseroid_gravity(const double *longitude, const double *latitude, const
 #define HMX_TESS(P, R)                                                                           \
     hipLaunchKernelGGL((tesseroid_kernel<P, R>), grid, dim3(kObsPerBlock), 0, s, longitude,     \
                        latitude, radius, (int)n_obs, roots, n_tesseroids, chunk_len, ratio,     \
-                       partial, overflow)
+                       density_cheb, partial, overflow)
     if (pot && radial_discretization) HMX_TESS(true, true);
     else if (pot) HMX_TESS(true, false);
     else if (radial_discretization) HMX_TESS(false, true);

@@ -192,20 +192,25 @@ int hmx_tesseroid_workspace_bytes(int64_t n_obs, int64_t n_tesseroids, size_t *b
  * and mGal conversion of g_z (tesseroid_gravity.py:207-211).
  *   longitude, latitude [deg], radius [m]: (n_obs) observation points;
  *   tesseroids: (n_tesseroids, 6) row-major [west, east, south, north,
- *     bottom, top] with west <= east (the Python layer applies the
- *     reference's checks and longitude continuity);
+ *     bottom, top], evaluated as given (the Python layer applies the
+ *     reference's checks and, with them, the longitude continuity);
  *   density_nodes: (n_tesseroids, 2) density at the lower and upper radial
  *     Gauss-Legendre node (both the tesseroid's density for a constant one);
- *     with radial_discretization the density must be constant;
+ *   density_cheb: NULL for constant densities, or (n_tesseroids, 32) the
+ *     Chebyshev coefficients of each tesseroid's density over its radial
+ *     extent [bottom, top] (sum_k c_k T_k((2 r - bottom - top) / (top -
+ *     bottom))), which give the densities at the radial nodes of the parts
+ *     that radial_discretization creates (the reference evaluates its
+ *     density function there, _tesseroid_variable_density.py);
  *   field: HMX_POTENTIAL or HMX_G_Z; out: (n_obs), overwritten;
  *   overflow: device int, set to 1 (stack of 100 exceeded) or 2 (more than
  *     100000 small tesseroids for a pair) as the reference's OverflowError;
  *     the caller clears it. */
 int hmx_tesseroid_gravity(const double *longitude, const double *latitude, const double *radius,
                           int64_t n_obs, const double *tesseroids, const double *density_nodes,
-                          int64_t n_tesseroids, uint32_t field, int radial_discretization,
-                          double *out, int *overflow, void *workspace, size_t workspace_bytes,
-                          void *stream);
+                          const double *density_cheb, int64_t n_tesseroids, uint32_t field,
+                          int radial_discretization, double *out, int *overflow, void *workspace,
+                          size_t workspace_bytes, void *stream);
 
 /* Replaces point_mass_cartesian / point_mass_spherical
  * (src/harmonica/_forward/point.py:357-454) with the choclo.point kernels

@@ -369,17 +369,20 @@ TESSEROID_RATIOS = {"potential": 1.0, "g_z": 2.5}
 WGS84_MEAN_RADIUS = 6371008.771415059
 
 
-def tesseroid_gravity(coordinates, tesseroids, density, field, radial=False, n_threads=0):
+def tesseroid_gravity(coordinates, tesseroids, density, field, radial=False, n_threads=0,
+                      continuity=True):
     """Oracle of ``harmonica.tesseroid_gravity`` (tesseroid_gravity.py:35-213)
     for constant densities, without the input checks: west > east boundaries
-    moved to [-180, 180) (_tesseroid_utils.py:_longitude_continuity), null
-    tesseroids discarded, g_z flipped and in mGal.  ``density`` may also be
+    moved to [-180, 180) (_tesseroid_utils.py:_longitude_continuity, part of
+    the checks; ``continuity=False`` leaves them as they are, as the
+    reference does with disable_checks=True), null tesseroids discarded, g_z
+    flipped and in mGal.  ``density`` may also be
     an (m, 2) array of densities at the lower and upper radial quadrature
     nodes of each tesseroid (variable density without radial
     discretisation).  Raises OverflowError as the reference does."""
     shape, (lon, lat, rad) = _coords(coordinates)
     t = _f64(np.atleast_2d(tesseroids)).copy()
-    swap = t[:, 0] > t[:, 1]
+    swap = (t[:, 0] > t[:, 1]) & bool(continuity)
     t[swap, 1] = ((t[swap, 1] + 180) % 360) - 180
     t[swap, 0] = ((t[swap, 0] + 180) % 360) - 180
     rho = np.asarray(density, dtype=np.float64)
@@ -415,6 +418,42 @@ def tesseroid_split_list(point, tesseroid, ratio, radial=False, stack_size=100, 
     if n < 0:
         raise OverflowError("discretisation overflow")
     return out[:n]
+
+
+def tesseroid_gravity_variable(coordinates, tesseroids, density, field, radial=False):
+    """Oracle of the variable-density loop (tesseroid_gravity.py:339-452)
+    for small cases, in Python: per (point, tesseroid) the small tesseroids
+    of the adaptive discretisation (tesseroid_split_list, the C
+    restatement), each integrated by 2 x 2 x 2 Gauss-Legendre nodes with the
+    density FUNCTION evaluated at its radial nodes
+    (_tesseroid_variable_density.py gauss_legendre_quadrature_variable_density)
+    and the spherical point-mass kernels (point.py:324-354) at the distance
+    of _forward/utils.py:163-201.  ``tesseroids`` must already be split by
+    the density-based discretisation (or need none).  g_z flipped, mGal."""
+    node = np.sqrt(1.0 / 3.0) * np.array([-1.0, 1.0])
+    ratio = TESSEROID_RATIOS[field]
+    lon, lat, rad = (np.atleast_1d(np.asarray(c, dtype=np.float64)).ravel() for c in coordinates[:3])
+    out = np.zeros(lon.size)
+    deg = np.pi / 180
+    for i in range(lon.size):
+        lon_r, sinphi, cosphi, r = lon[i] * deg, np.sin(lat[i] * deg), np.cos(lat[i] * deg), rad[i]
+        for t in np.atleast_2d(tesseroids):
+            for w, e, s, n, bottom, top in tesseroid_split_list((lon[i], lat[i], r), t, ratio, radial):
+                a = 1.0 / 8.0 * ((e - w) * deg) * ((n - s) * deg) * (top - bottom)
+                for lat_n in 0.5 * (n - s) * node + 0.5 * (n + s):
+                    sp, cp = np.sin(lat_n * deg), np.cos(lat_n * deg)
+                    for r_n in 0.5 * (top - bottom) * node + 0.5 * (top + bottom):
+                        mass = density(r_n) * a * r_n * r_n * cp
+                        for lon_n in 0.5 * (e - w) * node + 0.5 * (e + w):
+                            cospsi = sp * sinphi + cp * cosphi * np.cos(lon_n * deg - lon_r)
+                            dist = np.sqrt((r - r_n) ** 2 + 2.0 * r * r_n * (1.0 - cospsi))
+                            if field == "potential":
+                                out[i] += mass * GRAVITATIONAL_CONST / dist
+                            else:
+                                out[i] += mass * (-GRAVITATIONAL_CONST) * (r - r_n * cospsi) / dist**3
+    if field == "g_z":
+        out *= -1e5
+    return out
 
 
 def num_threads():

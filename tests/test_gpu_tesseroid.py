@@ -110,31 +110,37 @@ def test_spherical_shell_three_dim(hm, field, thickness):
     npt.assert_allclose(got, analytical, rtol=ACCURACY_THRESHOLD)
 
 
+@pytest.mark.parametrize("radial", [False, True])
 @pytest.mark.parametrize("field", ["potential", "g_z"])
 @pytest.mark.parametrize("thickness", [1e2, 1e3, 1e6])
-def test_spherical_shell_linear_density(hm, field, thickness):
-    """test_tesseroid_variable_density.py:440-478."""
+def test_spherical_shell_linear_density(hm, field, thickness, radial):
+    """test_tesseroid_variable_density.py:440-478, and the same with radial
+    discretisation for points 1 km above the shell (on its top the radial
+    splits never end, in the reference too: OverflowError)."""
     top = 6371e3
     bottom = top - thickness
     tess = shell_tesseroids(bottom, top, 12, 6, lon0=-180.0)
     density_outer, density_inner = 2500.0, 3300.0
     slope = (density_outer - density_inner) / (top - bottom)
     constant_term = density_outer - slope * top
-    coordinates = grid_coordinates((-180, 180, -90, 90), shape=(19, 13), extra=top)
+    height = top + 1e3 if radial else top
+    coordinates = grid_coordinates((-180, 180, -90, 90), shape=(19, 13), extra=height)
     constant = (np.pi * G * slope * (top**4 - bottom**4)
                 + 4 / 3.0 * np.pi * G * constant_term * (top**3 - bottom**3))
     potential = constant / coordinates[2]
     analytical = potential if field == "potential" else 1e5 * potential / coordinates[2]
     got = hm.tesseroid_gravity(coordinates, tess, lambda r: slope * r + constant_term,
-                               field=field)
+                               field=field, radial_adaptive_discretization=radial)
     npt.assert_allclose(got, analytical, rtol=ACCURACY_THRESHOLD)
 
 
+@pytest.mark.parametrize("radial", [False, True])
 @pytest.mark.parametrize("field", ["potential", "g_z"])
 @pytest.mark.parametrize("thickness", [1e2, 1e6])
 @pytest.mark.parametrize("b_factor", [5, 100])
-def test_spherical_shell_exponential_density(hm, field, thickness, b_factor):
-    """test_tesseroid_variable_density.py:481-525."""
+def test_spherical_shell_exponential_density(hm, field, thickness, b_factor, radial):
+    """test_tesseroid_variable_density.py:481-525, and the same with radial
+    discretisation for points 1 km above the shell."""
     top = 6371e3
     bottom = top - thickness
     tess = shell_tesseroids(bottom, top, 12, 6, lon0=-180.0)
@@ -145,7 +151,8 @@ def test_spherical_shell_exponential_density(hm, field, thickness, b_factor):
     def density(radius):
         return a_factor * np.exp(-b_factor * (radius - bottom) / thickness) + constant_term
 
-    coordinates = grid_coordinates((-180, 180, -90, 90), shape=(19, 13), extra=top)
+    height = top + 1e3 if radial else top
+    coordinates = grid_coordinates((-180, 180, -90, 90), shape=(19, 13), extra=height)
     radius = coordinates[2]
     k = b_factor / thickness
     potential = (4 * np.pi * G * a_factor / k**3 / radius
@@ -153,8 +160,36 @@ def test_spherical_shell_exponential_density(hm, field, thickness, b_factor):
                     - ((top * k) ** 2 + 2 * top * k + 2) * np.exp(-k * thickness))
                  + 4 / 3 * np.pi * G * constant_term * (top**3 - bottom**3) / radius)
     analytical = potential if field == "potential" else 1e5 * potential / radius
-    got = hm.tesseroid_gravity(coordinates, tess, density, field=field)
+    got = hm.tesseroid_gravity(coordinates, tess, density, field=field,
+                               radial_adaptive_discretization=radial)
     npt.assert_allclose(got, analytical, rtol=ACCURACY_THRESHOLD)
+
+
+@pytest.mark.parametrize("field", ["potential", "g_z"])
+def test_variable_density_radial_discretisation_vs_oracle(hm, field):
+    """Density functions with radial_adaptive_discretization=True: the GPU
+    evaluates the density at the parts' radial nodes from each tesseroid's
+    Chebyshev table; the oracle evaluates the function itself there
+    (oracle.tesseroid_gravity_variable) on the same pieces of the
+    density-based discretisation.  Points close to the tesseroids so that
+    they divide radially; a linear profile (interpolated exactly) and an
+    exponential one (interpolated to rounding)."""
+    tess = np.array([[-2.0, 2.0, -2.0, 2.0, R - 4e4, R - 1e4],
+                     [5.0, 8.0, -1.0, 3.0, R - 3e4, R - 2.5e4]])
+    rng = np.random.default_rng(17)
+    coords = (rng.uniform(-4, 10, 12), rng.uniform(-4, 5, 12), R + rng.uniform(1e2, 2e3, 12))
+    for density in (lambda r: 3000.0 - 0.01 * (r - (R - 4e4)),
+                    lambda r: 2700.0 + 100.0 * np.exp((r - R) / 2e4)):
+        from harmonica_amd import _tesseroid
+
+        pieces = _tesseroid.density_based_discretization(tess, density)
+        got = hm.tesseroid_gravity(coords, tess, density, field=field,
+                                   radial_adaptive_discretization=True)
+        expected = oracle.tesseroid_gravity_variable(coords, pieces, density, field, radial=True)
+        assert_parity(got, expected)
+        if field == "g_z":  # (for the potential, ratio 1, these points divide no tesseroid radially)
+            flat = hm.tesseroid_gravity(coords, tess, density, field=field)
+            assert np.abs(flat - got).max() > 1e-7 * np.abs(got).max()
 
 
 def test_variable_density_vs_oracle(hm):
@@ -195,6 +230,27 @@ def test_disable_checks(hm):
                                disable_checks=True)
     val = hm.tesseroid_gravity([0.0, 0.0, 10.0], valid, 100.0, field="potential")
     npt.assert_allclose(inv, -val)
+
+
+@pytest.mark.parametrize("field", ["potential", "g_z"])
+def test_west_greater_than_east_without_checks(hm, field):
+    """With disable_checks=True the reference skips _check_tesseroids and
+    with it the longitude continuity (tesseroid_gravity.py:171-174): a
+    tesseroid whose west boundary exceeds its east one is discretised and
+    integrated as given (the longitude steps are negative).  The GPU result
+    matches the oracle's loop on the tesseroids as given, and with the
+    checks on it matches the oracle after the continuity move."""
+    tess = np.array([[350.0, 10.0, -10.0, 10.0, R - 1e4, R],
+                     [340.0, -10.0, 20.0, 30.0, R - 2e4, R - 5e3],
+                     [-20.0, 15.0, -40.0, -30.0, R - 1e4, R]])
+    dens = np.array([1e3, 2e3, -5e2])
+    rng = np.random.default_rng(5)
+    coords = (rng.uniform(-180, 180, 60), rng.uniform(-60, 60, 60), np.full(60, R + 5e4))
+    raw = hm.tesseroid_gravity(coords, tess, dens, field=field, disable_checks=True)
+    assert_parity(raw, oracle.tesseroid_gravity(coords, tess, dens, field, continuity=False))
+    moved = hm.tesseroid_gravity(coords, tess, dens, field=field)
+    assert_parity(moved, oracle.tesseroid_gravity(coords, tess, dens, field))
+    assert np.abs(raw - moved).max() > 1e-3 * np.abs(moved).max()
 
 
 def test_stack_overflow_raises(hm):

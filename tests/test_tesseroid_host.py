@@ -294,3 +294,55 @@ def test_nonans_mask_and_thin_discard():
     t, d = _discard_thin_tesseroids(tess, np.array([1.0, 2.0, 3.0]), 0.5)
     npt.assert_array_equal(t, tess[:2])
     npt.assert_array_equal(d, [1.0, 2.0])
+
+
+@pytest.mark.parametrize("profile", ["linear", "cubic", "exponential", "steep exponential"])
+def test_chebyshev_density_tables(profile):
+    """The per-tesseroid Chebyshev tables that carry a density function to
+    the GPU for radial discretisation: evaluated at random radii of each
+    tesseroid (numpy's chebval on the mapped radius), they match the
+    function to within 1e-13 of its largest value (polynomials up to degree
+    31 exactly, exponentials on the pieces of the density-based
+    discretisation to rounding)."""
+    from numpy.polynomial import chebyshev
+
+    from harmonica_amd import _tesseroid
+
+    R = 6371e3
+    rng = np.random.default_rng(4)
+    bottom = R - rng.uniform(1e2, 1e6, 20)
+    top = bottom + rng.uniform(10.0, 3e5, 20)
+    tess = np.column_stack([np.zeros(20), np.ones(20), np.zeros(20), np.ones(20), bottom, top])
+    density = {
+        "linear": lambda r: 3300.0 - 0.003 * (r - R),
+        "cubic": lambda r: 2000.0 + 1e-15 * (r - R + 5e5) ** 3,
+        "exponential": lambda r: 2670.0 + 630.0 * np.exp(-5 * (r - R + 1e6) / 1e6),
+        "steep exponential": lambda r: 2670.0 + 630.0 * np.exp(-100 * (r - R + 1e6) / 1e6),
+    }[profile]
+    # the pieces of the density-based discretisation, as tesseroid_gravity
+    # tabulates them
+    tess = _tesseroid.density_based_discretization(tess, density)
+    bottom, top = tess[:, 4], tess[:, 5]
+    c = _tesseroid._chebyshev_densities(tess, density)
+    assert c.shape == (tess.shape[0], _tesseroid.CHEB_N)
+    for q in range(tess.shape[0]):
+        r = rng.uniform(bottom[q], top[q], 50)
+        x = (2 * r - (bottom[q] + top[q])) / (top[q] - bottom[q])
+        f = np.array([density(v) for v in r])
+        npt.assert_allclose(chebyshev.chebval(x, c[q]), f, rtol=0, atol=1e-13 * np.abs(f).max())
+
+
+@pytest.mark.parametrize("radial", [False, True])
+@pytest.mark.parametrize("field", ["potential", "g_z"])
+def test_variable_density_oracle_with_constant_function(field, radial):
+    """The Python variable-density oracle with a constant density function
+    equals the C oracle's constant-density loop (itself pinned by the
+    analytical shells above)."""
+    R = oracle.WGS84_MEAN_RADIUS
+    tess = np.array([[-2.0, 2.0, -2.0, 2.0, R - 4e4, R - 1e4],
+                     [5.0, 8.0, -1.0, 3.0, R - 3e4, R - 2.5e4]])
+    rng = np.random.default_rng(3)
+    coords = (rng.uniform(-4, 10, 6), rng.uniform(-4, 5, 6), R + rng.uniform(1e2, 2e3, 6))
+    got = oracle.tesseroid_gravity_variable(coords, tess, lambda r: 2670.0, field, radial=radial)
+    expected = oracle.tesseroid_gravity(coords, tess, np.full(2, 2670.0), field, radial=radial)
+    npt.assert_allclose(got, expected, rtol=1e-12, atol=1e-13 * np.abs(expected).max())
