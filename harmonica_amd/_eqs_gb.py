@@ -81,6 +81,30 @@ def _get_region_data_sources(coordinates, points):
     )
 
 
+#: Windows factorised per batch in the FP64 mode with damping: None picks
+#: the largest batch whose buffers fit half of the free device memory (at
+#: most all windows); 1 fits window by window.
+FACTOR_BATCH = None
+
+
+def _factor_batch(n, max_wd, max_ws, mode, damping, n_windows):
+    """Windows per batch of the batched factorisation (1: none)."""
+    if FACTOR_BATCH is not None:
+        return max(1, int(FACTOR_BATCH))
+    if mode != _lib.HMX_FIT_FP64 or not damping > 0 or n_windows < 2:
+        return 1
+    sizes = []
+    for batch in (1, 2):
+        nbytes = ctypes.c_size_t(0)
+        check(lib().hmx_eqs_gb_fit_workspace_bytes(n, max_wd, max_ws, mode, damping, batch,
+                                                   ctypes.byref(nbytes)))
+        sizes.append(nbytes.value)
+    per_window = max(sizes[1] - sizes[0], 1) / 2
+    free, _ = torch.cuda.mem_get_info()
+    fits = int((0.5 * free - sizes[0]) // per_window)
+    return max(1, min(n_windows, fits))
+
+
 class EquivalentSourcesGB(EquivalentSources):
     r"""
     Gradient-boosted equivalent sources (reference:
@@ -175,8 +199,11 @@ class EquivalentSourcesGB(EquivalentSources):
         rtol = 1e-12 if mode == _lib.HMX_FIT_FP64 else MIXED_RTOL
         if self.damping is None:  # LinearRegression's minimum-norm least squares
             mode |= _lib.HMX_FIT_NO_DAMPING
+        damping = float(0.0 if self.damping is None else self.damping)
+        batch = _factor_batch(n, max_wd, max_ws, mode, damping, n_windows)
         nbytes = ctypes.c_size_t(0)
-        check(lib().hmx_eqs_gb_fit_workspace_bytes(n, max_wd, max_ws, mode, ctypes.byref(nbytes)))
+        check(lib().hmx_eqs_gb_fit_workspace_bytes(n, max_wd, max_ws, mode, damping, batch,
+                                                   ctypes.byref(nbytes)))
         ws = torch.empty(nbytes.value, dtype=torch.uint8, device=dev)
         coefs = torch.empty(m, dtype=torch.float64, device=dev)
         residue = torch.empty(n, dtype=torch.float64, device=dev)
@@ -186,15 +213,15 @@ class EquivalentSourcesGB(EquivalentSources):
         check(lib().hmx_eqs_gb_fit(
             *(ptr(x) for x in c), n, ptr(d), ptr(w), *(ptr(x) for x in p), m, ptr(didx),
             data_offset.ctypes.data_as(i64p), ptr(sidx), source_offset.ctypes.data_as(i64p),
-            n_windows, float(0.0 if self.damping is None else self.damping), mode, max_iter,
-            rtol, ptr(coefs), ptr(residue), ptr(rmse), info, ptr(ws), nbytes.value,
-            stream_handle(None)))
+            n_windows, damping, mode, batch, max_iter, rtol, ptr(coefs), ptr(residue), ptr(rmse),
+            info, ptr(ws), nbytes.value, stream_handle(None)))
         self.coefs_ = coefs.cpu().numpy().astype(self.points_[0].dtype, copy=False)
         self.rmse_per_iteration_ = rmse.cpu().numpy()
         # summed over the windows, except the last update and the largest
         # shift and diagonal; "svd": windows solved by the SVD route
         self.fit_info_ = dict(zip(FIT_INFO_KEYS, list(info)))
         self.fit_info_["windows"] = n_windows
+        self.fit_info_["factor_batch"] = batch
 
     def _create_windows(self, coordinates, shuffle=True):
         """Source and data indices of each window (gradient_boosted.py:285-369)."""

@@ -75,9 +75,9 @@ SIGNATURES = {
     "hmx_tesseroid_gravity": (_int, [_vp, _vp, _vp, _i64, _vp, _vp, _vp, _i64, _u32, _int, _vp,
                                      _vp, _vp, ctypes.c_size_t, _vp]),
     "hmx_geocentric": (_int, [_vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp]),
-    "hmx_eqs_gb_fit_workspace_bytes": (_int, [_i64, _i64, _i64, _int, _szp]),
+    "hmx_eqs_gb_fit_workspace_bytes": (_int, [_i64, _i64, _i64, _int, _dbl, _i64, _szp]),
     "hmx_eqs_gb_fit": (_int, [_vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp,
-                              _vp, _i64, _dbl, _int, _int, _dbl, _vp, _vp, _vp,
+                              _vp, _i64, _dbl, _int, _i64, _int, _dbl, _vp, _vp, _vp,
                               ctypes.POINTER(ctypes.c_double), _vp, ctypes.c_size_t, _vp]),
     "hmx_gz_grad_workspace_bytes": (_int, [_i64, _i64, _szp]),
     "hmx_prism_gz_jvp": (_int, [_vp, _vp, _vp, _i64, _vp, _vp, _vp, _i64, _vp, _vp, ctypes.c_size_t,

@@ -441,8 +441,13 @@ template <class T, bool UPPER>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void gemm_tn_kernel(Panel<T> A, Panel<T> B, int64_t K,
                                                       int64_t M, int64_t N, double *C, int64_t ldc,
                                                       double beta, double sign, double shift,
-                                                      int64_t tiles_n) {
+                                                      int64_t tiles_n, int64_t bsa, int64_t bsb,
+                                                      int64_t bsc) {
     __shared__ T lds[2][2][kKC * kPad];
+    // problem blockIdx.z of a strided batch (element strides of A, B, C)
+    A.X += (int64_t)blockIdx.z * bsa;
+    B.X += (int64_t)blockIdx.z * bsb;
+    C += (int64_t)blockIdx.z * bsc;
     int64_t rb, cb;
     if (UPPER) {  // row-major over the upper triangle (FP64 MFMA is compute-bound)
         int64_t bq = blockIdx.x, rem = tiles_n;
@@ -465,20 +470,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void g
         tile_k_loop<T, false, UPPER>(A, B, K, r0, c0, lds, M, N, C, ldc, beta, sign, shift);
 }
 
+// batch > 1: a strided batch of equal-shaped problems (element strides
+// bsa, bsb, bsc between consecutive A, B, C; multiples of 2 keep the
+// 16-byte loads aligned).
 template <class T>
 int launch_gemm(const T *A, int64_t lda, const T *B, int64_t ldb, int64_t K, int64_t M,
                 int64_t N, double *C, int64_t ldc, double beta, double sign, double shift,
-                bool upper, hipStream_t s) {
-    if (M <= 0 || N <= 0) return HMX_OK;
+                bool upper, hipStream_t s, int batch = 1, int64_t bsa = 0, int64_t bsb = 0,
+                int64_t bsc = 0) {
+    if (M <= 0 || N <= 0 || batch <= 0) return HMX_OK;
     const int64_t tm = (M + kST - 1) / kST, tn = (N + kST - 1) / kST;
     const Panel<T> pa = make_panel(A, lda, K, M), pb = make_panel(B, ldb, K, N);
     if (upper) {
-        hipLaunchKernelGGL((gemm_tn_kernel<T, true>), dim3((unsigned)(tn * (tn + 1) / 2)),
-                           dim3(256), 0, s, pa, pb, K, M, N, C, ldc, beta, sign, shift, tn);
+        hipLaunchKernelGGL((gemm_tn_kernel<T, true>),
+                           dim3((unsigned)(tn * (tn + 1) / 2), 1, (unsigned)batch), dim3(256), 0,
+                           s, pa, pb, K, M, N, C, ldc, beta, sign, shift, tn, bsa, bsb, bsc);
         HMX_LAUNCHED("gemm_tn_kernel<upper>");
     } else {
-        hipLaunchKernelGGL((gemm_tn_kernel<T, false>), dim3((unsigned)(tm * tn)), dim3(256), 0,
-                           s, pa, pb, K, M, N, C, ldc, beta, sign, shift, tn);
+        hipLaunchKernelGGL((gemm_tn_kernel<T, false>), dim3((unsigned)(tm * tn), 1, (unsigned)batch),
+                           dim3(256), 0, s, pa, pb, K, M, N, C, ldc, beta, sign, shift, tn, bsa,
+                           bsb, bsc);
         HMX_LAUNCHED("gemm_tn_kernel");
     }
     return HMX_OK;
@@ -486,14 +497,18 @@ int launch_gemm(const T *A, int64_t lda, const T *B, int64_t ldb, int64_t K, int
 
 template <class T>
 int launch_syrk(const T *X, int64_t ldx, int64_t K, int64_t m, double *C, int64_t ldc, double beta,
-                double sign, double shift, hipStream_t s) {
-    return launch_gemm<T>(X, ldx, X, ldx, K, m, m, C, ldc, beta, sign, shift, true, s);
+                double sign, double shift, hipStream_t s, int batch = 1, int64_t bsx = 0,
+                int64_t bsc = 0) {
+    return launch_gemm<T>(X, ldx, X, ldx, K, m, m, C, ldc, beta, sign, shift, true, s, batch, bsx,
+                          bsx, bsc);
 }
 
 int launch_gemm_tn(const double *A, int64_t lda, const double *B, int64_t ldb, int64_t K,
                    int64_t M, int64_t N, double *C, int64_t ldc, double beta, double sign,
-                   hipStream_t s) {
-    return launch_gemm<double>(A, lda, B, ldb, K, M, N, C, ldc, beta, sign, 0.0, false, s);
+                   hipStream_t s, int batch = 1, int64_t bsa = 0, int64_t bsb = 0,
+                   int64_t bsc = 0) {
+    return launch_gemm<double>(A, lda, B, ldb, K, M, N, C, ldc, beta, sign, 0.0, false, s, batch,
+                               bsa, bsb, bsc);
 }
 
 // ---------------------------------------------------------------- Cholesky
@@ -528,8 +543,14 @@ __global__ __launch_bounds__(1024) void potrf_diag_kernel(double *__restrict__ D
                                                           int nb, int64_t col0,
                                                           double *__restrict__ Dinv,
                                                           double *__restrict__ DinvT,
-                                                          int *__restrict__ info) {
+                                                          int *__restrict__ info, int64_t bsd,
+                                                          int64_t bsinv, int info_stride) {
     constexpr int S = kNB + 1;
+    // problem blockIdx.x of a strided batch
+    D += (int64_t)blockIdx.x * bsd;
+    Dinv += (int64_t)blockIdx.x * bsinv;
+    DinvT += (int64_t)blockIdx.x * bsinv;
+    info += (int64_t)blockIdx.x * info_stride;
     __shared__ double L[kNB * S];
     __shared__ double piv[kNB];
     const int tid = threadIdx.x, tx = tid & 31, ty = tid >> 5;  // 32 x 32 threads
@@ -908,6 +929,11 @@ __global__ void div_kernel(int64_t n, const double *__restrict__ x, const double
     if (i < n) y[i] = x[i] / s[i];
 }
 
+__global__ void fill_zero_f64_kernel(double *__restrict__ x, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = 0.0;
+}
+
 __global__ void copy_kernel(int64_t n, const double *__restrict__ x, double *__restrict__ y) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) y[i] = x[i];
@@ -958,9 +984,6 @@ __global__ void add_diag_kernel(double *__restrict__ A, int64_t m, double tau) {
     if (j < m) A[j * m + j] += tau;
 }
 
-__global__ void zero_int_kernel(int *p) {
-    if (threadIdx.x == 0) *p = 0;
-}
 
 // ------------------------------------------ split-precision normal matrix
 // Exact-product schemes for X^T X on low-precision MFMA, in two slice
@@ -1460,60 +1483,75 @@ struct Fit {
         return jt(w, t, alpha, p, out);
     }
 
+    // Dimension of the factor (m unless padded; see the batched fits).
+    int64_t mf = 0;
+    int64_t fm() const { return mf ? mf : m; }
+
     // Blocked right-looking factorisation A = U^T U (upper, in place) in two
     // levels: within an outer block of kOuter rows, kNB-row steps factor the
     // diagonal block, solve their row panel up to column m and update only
     // the remaining rows of the outer block; then one rank-kOuter update
     // (K = kOuter, the regime where the MFMA kernel is efficient) brings the
-    // trailing matrix up to date.
-    int cholesky(int *pivot) {
-        hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(1), 0, s, dinfo);
-        HMX_LAUNCHED("zero_int_kernel");
-        for (int64_t P = 0; P < m; P += kOuter) {
-            const int64_t Pend = P + kOuter < m ? P + kOuter : m;
+    // trailing matrix up to date.  batch > 1: the same on a strided batch of
+    // matrices (element strides bs_a between them and bs_inv between their
+    // diagonal-block inverses), every launch covering the whole batch;
+    // pivot then receives one status per matrix (0 or the 1-based column of
+    // its failed pivot, from the device array infos).
+    int cholesky(int *pivot, int batch = 1, int64_t bs_a = 0, int64_t bs_inv = 0,
+                 int *infos = nullptr) {
+        const int64_t n = fm();
+        int *dev_info = batch > 1 ? infos : dinfo;
+        hipLaunchKernelGGL(fill_int_kernel, dim3(blocks_for(batch)), dim3(256), 0, s, dev_info,
+                           (int64_t)batch, 0);
+        HMX_LAUNCHED("fill_int_kernel");
+        for (int64_t P = 0; P < n; P += kOuter) {
+            const int64_t Pend = P + kOuter < n ? P + kOuter : n;
             for (int64_t p = P; p < Pend; p += kNB) {
-                const int nb = (int)(m - p < kNB ? m - p : kNB);
-                double *D = A + p * m + p;
+                const int nb = (int)(n - p < kNB ? n - p : kNB);
+                double *D = A + p * n + p;
                 const int64_t blk = p / kNB;
-                hipLaunchKernelGGL(potrf_diag_kernel, dim3(1), dim3(1024), 0, s, D, m, nb, p,
-                                   Dinv + blk * kNB * kNB, DinvT + blk * kNB * kNB, dinfo);
+                hipLaunchKernelGGL(potrf_diag_kernel, dim3((unsigned)batch), dim3(1024), 0, s, D, n,
+                                   nb, p, Dinv + blk * kNB * kNB, DinvT + blk * kNB * kNB,
+                                   dev_info, bs_a, bs_inv, batch > 1 ? 1 : 0);
                 HMX_LAUNCHED("potrf_diag_kernel");
-                const int64_t rest = m - p - nb;
+                const int64_t rest = n - p - nb;
                 if (rest <= 0) break;
-                // U12 = U11^-T A12 = (U11^-1)^T A12, in place, up to column m.
-                int st = launch_gemm_tn(Dinv + blk * kNB * kNB, kNB, D + nb, m, nb, nb, rest,
-                                        D + nb, m, 0.0, 1.0, s);
+                // U12 = U11^-T A12 = (U11^-1)^T A12, in place, up to column n.
+                int st = launch_gemm_tn(Dinv + blk * kNB * kNB, kNB, D + nb, n, nb, nb, rest,
+                                        D + nb, n, 0.0, 1.0, s, batch, bs_inv, bs_a, bs_a);
                 if (st) return st;
-                // Rows [p + nb, Pend) of the outer block, columns [p + nb, m)
+                // Rows [p + nb, Pend) of the outer block, columns [p + nb, n)
                 // (the few entries below the diagonal that this rectangle
                 // covers are never read).
                 const int64_t rows_in = Pend - p - nb;
                 if (rows_in > 0) {
-                    st = launch_gemm<double>(D + nb, m, D + nb, m, nb, rows_in, rest,
-                                             D + nb * m + nb, m, 1.0, -1.0, 0.0, false, s);
+                    st = launch_gemm<double>(D + nb, n, D + nb, n, nb, rows_in, rest,
+                                             D + nb * n + nb, n, 1.0, -1.0, 0.0, false, s, batch,
+                                             bs_a, bs_a, bs_a);
                     if (st) return st;
                 }
-                if (blk % 16 == 15) {  // stop early once a pivot has failed
+                if (batch == 1 && blk % 16 == 15) {  // stop early once a pivot has failed
                     HMX_HIP(hipMemcpyAsync(pivot, dinfo, sizeof(int), hipMemcpyDeviceToHost, s));
                     HMX_HIP(hipStreamSynchronize(s));
                     if (*pivot) return HMX_OK;
                 }
             }
-            if (Pend < m) {
-                const int64_t K = Pend - P, rest = m - Pend;
-                int st = launch_syrk<double>(A + P * m + Pend, m, K, rest, A + Pend * m + Pend, m,
-                                             1.0, -1.0, 0.0, s);
+            if (Pend < n) {
+                const int64_t K = Pend - P, rest = n - Pend;
+                int st = launch_syrk<double>(A + P * n + Pend, n, K, rest, A + Pend * n + Pend, n,
+                                             1.0, -1.0, 0.0, s, batch, bs_a, bs_a);
                 if (st) return st;
             }
         }
-        HMX_HIP(hipMemcpyAsync(pivot, dinfo, sizeof(int), hipMemcpyDeviceToHost, s));
+        HMX_HIP(hipMemcpyAsync(pivot, dev_info, (size_t)batch * sizeof(int), hipMemcpyDeviceToHost,
+                               s));
         HMX_HIP(hipStreamSynchronize(s));
         return HMX_OK;
     }
 
     // Flags and error word at zero, before the first solve.
     int init_solves() {
-        const int64_t n = 2 * ((m + kNB - 1) / kNB) + 1;
+        const int64_t n = 2 * ((fm() + kNB - 1) / kNB) + 1;
         hipLaunchKernelGGL(fill_int_kernel, dim3(blocks_for(n)), dim3(256), 0, s, flags, n, 0);
         HMX_LAUNCHED("fill_int_kernel");
         terr = flags + n - 1;
@@ -1524,14 +1562,86 @@ struct Fit {
     // x = A^-1 r with the factor: U^T y = r (forward), then U x = y
     // (backward), one launch each (y is scratch; r is not modified).
     int solve(double *r, double *y, double *x) {
-        const int nb = (int)((m + kNB - 1) / kNB);
+        const int64_t n = fm();
+        const int nb = (int)((n + kNB - 1) / kNB);
         ++epoch;
-        hipLaunchKernelGGL(trsv_fwd_sync_kernel, dim3(nb), dim3(1024), 0, s, A, m, m, Dinv, r, y,
+        hipLaunchKernelGGL(trsv_fwd_sync_kernel, dim3(nb), dim3(1024), 0, s, A, n, n, Dinv, r, y,
                            flags, epoch, terr);
         HMX_LAUNCHED("trsv_fwd_sync_kernel");
-        hipLaunchKernelGGL(trsv_bwd_sync_kernel, dim3(nb), dim3(1024), 0, s, A, m, m, DinvT, y, x,
+        hipLaunchKernelGGL(trsv_bwd_sync_kernel, dim3(nb), dim3(1024), 0, s, A, n, n, DinvT, y, x,
                            flags + nb, epoch, terr);
         HMX_LAUNCHED("trsv_bwd_sync_kernel");
+        return HMX_OK;
+    }
+
+    // x = (factor)^-1 b, then preconditioned CG on the exact FP64 operator
+    // A = S J^T W J S + alpha I (starting from x), stopping when the
+    // data-space update |a| |J S p| falls below rtol |J S x| or after
+    // `allowed` iterations.  The vectors hold fm() doubles (the factor's
+    // dimension); entries past m stay zero.
+    int solve_refine(const double *b, double *x, double *r, double *z, double *pd, double *q,
+                     double *wk, double *wk2, int allowed, double rtol, int &iters,
+                     double &last_rel) {
+        const unsigned gm = blocks_for(m);
+        iters = 0;
+        last_rel = 0.0;
+        int st = init_solves();
+        if (st) return st;
+        if (fm() > m) {  // the factor's padding: zero right-hand sides
+            hipLaunchKernelGGL(fill_zero_f64_kernel, dim3(blocks_for(fm())), dim3(256), 0, s, wk,
+                               fm());
+            HMX_LAUNCHED("fill_zero_f64_kernel");
+        }
+        hipLaunchKernelGGL(copy_kernel, dim3(gm), dim3(256), 0, s, m, b, wk);
+        HMX_LAUNCHED("copy_kernel");
+        st = solve(wk, wk2, x);
+        if (st) return st;
+        if (allowed <= 0) return HMX_OK;
+        double *t_dat = tmp;
+        st = apply(x, t_dat, q);  // q = A x
+        if (st) return st;
+        hipLaunchKernelGGL(sub_kernel, dim3(gm), dim3(256), 0, s, m, b, q, r);  // r = b - A x
+        HMX_LAUNCHED("sub_kernel");
+        double ynorm = 0.0;
+        st = reduce(0, t_dat, t_dat, nd, 1, &ynorm);  // |J S x|^2
+        if (st) return st;
+        hipLaunchKernelGGL(copy_kernel, dim3(gm), dim3(256), 0, s, m, r, wk);
+        HMX_LAUNCHED("copy_kernel");
+        st = solve(wk, wk2, z);
+        if (st) return st;
+        hipLaunchKernelGGL(copy_kernel, dim3(gm), dim3(256), 0, s, m, z, pd);
+        HMX_LAUNCHED("copy_kernel");
+        double rz = 0.0;
+        st = reduce(0, r, z, m, 1, &rz);
+        if (st) return st;
+        for (iters = 1; iters <= allowed; ++iters) {
+            st = apply(pd, t_dat, q);  // q = A p, t = J S p
+            if (st) return st;
+            double pq = 0.0, tt = 0.0;
+            st = reduce(0, pd, q, m, 1, &pq);
+            if (st) return st;
+            st = reduce(0, t_dat, t_dat, nd, 1, &tt);
+            if (st) return st;
+            if (!(pq > 0.0)) break;
+            const double a = rz / pq;
+            hipLaunchKernelGGL(axpy_kernel, dim3(gm), dim3(256), 0, s, m, a, pd, x);
+            HMX_LAUNCHED("axpy_kernel");
+            hipLaunchKernelGGL(axpy_kernel, dim3(gm), dim3(256), 0, s, m, -a, q, r);
+            HMX_LAUNCHED("axpy_kernel");
+            last_rel = std::fabs(a) * std::sqrt(tt) / std::sqrt(ynorm > 0 ? ynorm : 1.0);
+            if (last_rel <= rtol) break;
+            hipLaunchKernelGGL(copy_kernel, dim3(gm), dim3(256), 0, s, m, r, wk);
+            HMX_LAUNCHED("copy_kernel");
+            st = solve(wk, wk2, z);
+            if (st) return st;
+            double rz_new = 0.0;
+            st = reduce(0, r, z, m, 1, &rz_new);
+            if (st) return st;
+            hipLaunchKernelGGL(xpby_kernel, dim3(gm), dim3(256), 0, s, m, z, rz_new / rz, pd);
+            HMX_LAUNCHED("xpby_kernel");
+            rz = rz_new;
+        }
+        if (iters > allowed) iters = allowed;
         return HMX_OK;
     }
 
@@ -1572,6 +1682,162 @@ __global__ void mfma_probe_kernel(int dtype, const double *__restrict__ a,
 }
 
 }  // namespace
+
+// ------------------------------------------- batched window factorisation
+// The gradient-boosted fit's windows in batches: the normal matrices of a
+// batch of windows are formed and factorised together (every launch covers
+// the batch), and each window is then solved from its stored factor in the
+// sequential window loop.  The windows' matrices are padded to a common
+// size: m_pad = the largest window's sources rounded up to 8, the scaled
+// jacobians to n_max rows (zero rows and columns), and the shift added to
+// the whole diagonal (damping > 0), so a padding column's factor row is
+// sqrt(damping) on the diagonal and zero elsewhere and solves give zeros
+// there.
+struct BatchLayout {
+    int64_t m_pad, n_max, nblk;
+    size_t x_off, a_off, dinv_off, scale_off, vec_off, tmp_off, part_off, split_off, dinfo_off,
+        flags_off, infos_off, total;
+};
+
+inline size_t al256(size_t b) { return (b + 255) / 256 * 256; }
+
+BatchLayout batch_layout(int64_t n_max, int64_t m_max, int64_t batch) {
+    BatchLayout L;
+    L.m_pad = (m_max + 7) / 8 * 8;
+    L.n_max = n_max;
+    L.nblk = (L.m_pad + kNB - 1) / kNB;
+    const size_t B = (size_t)batch, mp = (size_t)L.m_pad;
+    size_t off = 0;
+    L.x_off = off;
+    off += al256(B * (size_t)n_max * mp * sizeof(double));
+    L.a_off = off;
+    off += al256(B * mp * mp * sizeof(double));
+    L.dinv_off = off;
+    off += al256(2 * B * (size_t)L.nblk * kNB * kNB * sizeof(double));
+    L.scale_off = off;
+    off += al256(B * mp * sizeof(double));
+    L.vec_off = off;  // 9 vectors of m_pad (b, x, r, z, p, q, wk, wk2, spare)
+    off += al256(9 * mp * sizeof(double));
+    L.tmp_off = off;
+    off += al256((size_t)n_max * sizeof(double));
+    L.part_off = off;
+    off += al256(256 * sizeof(double));
+    L.split_off = off;
+    off += al256(split_doubles(n_max > L.m_pad ? n_max : L.m_pad) * sizeof(double));
+    L.dinfo_off = off;
+    off += 256;
+    L.flags_off = off;
+    off += al256(solve_flag_bytes(L.m_pad));
+    L.infos_off = off;
+    off += al256(B * sizeof(int));
+    L.total = off;
+    return L;
+}
+
+size_t eqs_batch_bytes(int64_t n_max, int64_t m_max, int64_t batch) {
+    return batch_layout(n_max, m_max, batch).total;
+}
+
+namespace {
+
+// Fit of window w with the batch buffers (vectors and factor of slot b).
+Fit batch_fit(const EqsWindow &w, double damping, const BatchLayout &L, char *buf, int64_t b,
+              hipStream_t s) {
+    Fit f;
+    f.e = w.e;
+    f.n = w.n;
+    f.u = w.u;
+    f.w = w.w;
+    f.nd = w.nd;
+    f.pe = w.pe;
+    f.pn = w.pn;
+    f.pu = w.pu;
+    f.m = w.m;
+    f.mf = L.m_pad;
+    f.alpha = damping;
+    f.s = s;
+    f.scale = reinterpret_cast<double *>(buf + L.scale_off) + b * L.m_pad;
+    f.A = reinterpret_cast<double *>(buf + L.a_off) + b * L.m_pad * L.m_pad;
+    f.Dinv = reinterpret_cast<double *>(buf + L.dinv_off) + b * L.nblk * kNB * kNB;
+    f.DinvT = f.Dinv + (int64_t)0;  // set by the caller (batch stride)
+    f.tmp = reinterpret_cast<double *>(buf + L.tmp_off);
+    f.part = reinterpret_cast<double *>(buf + L.part_off);
+    f.split = reinterpret_cast<double *>(buf + L.split_off);
+    f.dinfo = reinterpret_cast<int *>(buf + L.dinfo_off);
+    f.flags = reinterpret_cast<int *>(buf + L.flags_off);
+    return f;
+}
+
+}  // namespace
+
+int eqs_factor_batch(const EqsWindow *wins, int64_t batch, int64_t n_max, int64_t m_max,
+                     double damping, void *buffer, size_t buffer_bytes, int *status,
+                     hipStream_t s) {
+    if (batch <= 0) return HMX_OK;
+    if (!(damping > 0.0)) return fail(HMX_ERR_ARG, "batched factorisation needs damping > 0");
+    const BatchLayout L = batch_layout(n_max, m_max, batch);
+    if (!buffer || buffer_bytes < L.total)
+        return fail(HMX_ERR_ARG, "batch buffer too small: %zu < %zu bytes", buffer_bytes, L.total);
+    char *buf = static_cast<char *>(buffer);
+    double *X = reinterpret_cast<double *>(buf + L.x_off);
+    const int64_t bs_x = L.n_max * L.m_pad, bs_a = L.m_pad * L.m_pad;
+    const int64_t bs_inv = L.nblk * kNB * kNB;
+    hipLaunchKernelGGL(fill_zero_f64_kernel, dim3(blocks_for(batch * bs_x)), dim3(256), 0, s, X,
+                       batch * bs_x);
+    HMX_LAUNCHED("fill_zero_f64_kernel");
+    // column scales and the scaled jacobian of each window (zero padding)
+    for (int64_t b = 0; b < batch; ++b) {
+        const EqsWindow &w = wins[b];
+        if (w.nd <= 0 || w.m <= 0 || w.nd > n_max || w.m > m_max)
+            return fail(HMX_ERR_ARG, "window %lld of the batch has a bad size", (long long)b);
+        Fit f = batch_fit(w, damping, L, buf, b, s);
+        int st = f.col_stats();
+        if (st) return st;
+        const unsigned gm = blocks_for(w.m), gy = (unsigned)(w.nd < 8192 ? w.nd : 8192);
+        hipLaunchKernelGGL(scaled_jacobian_kernel<double>, dim3(gm, gy), dim3(256), 0, s, w.e,
+                           w.n, w.u, w.w, w.nd, w.pe, w.pn, w.pu, f.scale, w.m, X + b * bs_x,
+                           L.m_pad);
+        HMX_LAUNCHED("scaled_jacobian_kernel");
+    }
+    // A_b = X_b^T X_b + damping I for the whole batch, then the factors
+    double *A = reinterpret_cast<double *>(buf + L.a_off);
+    int st = launch_syrk<double>(X, L.m_pad, L.n_max, L.m_pad, A, L.m_pad, 0.0, 1.0, damping, s,
+                                 (int)batch, bs_x, bs_a);
+    if (st) return st;
+    Fit f = batch_fit(wins[0], damping, L, buf, 0, s);
+    f.m = L.m_pad;
+    f.DinvT = f.Dinv + batch * bs_inv;
+    return f.cholesky(status, (int)batch, bs_a, bs_inv,
+                      reinterpret_cast<int *>(buf + L.infos_off));
+}
+
+int eqs_solve_factored(const EqsWindow &w, const double *data, double damping, void *buffer,
+                       int64_t b, int64_t batch, int64_t n_max, int64_t m_max, int max_iter,
+                       double rtol, double *coefs, double *info, hipStream_t s) {
+    const BatchLayout L = batch_layout(n_max, m_max, batch);
+    char *buf = static_cast<char *>(buffer);
+    Fit f = batch_fit(w, damping, L, buf, b, s);
+    f.DinvT = f.Dinv + batch * L.nblk * kNB * kNB;
+    double *vec = reinterpret_cast<double *>(buf + L.vec_off);
+    const int64_t mp = L.m_pad;
+    double *bv = vec, *x = vec + mp, *r = x + mp, *z = r + mp, *pd = z + mp, *q = pd + mp,
+           *wk = q + mp, *wk2 = wk + mp;
+    int st = f.jt(w.w, data, 0.0, nullptr, bv);  // b = S J^T W d
+    if (st) return st;
+    int iters = 0;
+    double last_rel = 0.0;
+    st = f.solve_refine(bv, x, r, z, pd, q, wk, wk2, max_iter, rtol, iters, last_rel);
+    if (st) return st;
+    hipLaunchKernelGGL(div_kernel, dim3(blocks_for(w.m)), dim3(256), 0, s, w.m, x, f.scale, coefs);
+    HMX_LAUNCHED("div_kernel");
+    st = f.check_solves();
+    if (st) return st;
+    for (int k = 0; k < 11; ++k) info[k] = 0.0;
+    info[0] = (double)iters;
+    info[1] = last_rel;
+    return HMX_OK;
+}
+
 }  // namespace hmx
 
 extern "C" {
@@ -1891,72 +2157,17 @@ int hmx_eqs_fit(const double *easting, const double *northing, const double *upw
         if (!(umin * umin >= kLstsqPivotRatio * umax * umax)) use_svd = true;
     }
 
-    // 3. x = (factor)^-1 b.
-    if (!use_svd) {
-        st = f.init_solves();
-        if (st) return st;
-        hipLaunchKernelGGL(copy_kernel, dim3(gm), dim3(256), 0, s, m, b, wk);
-        HMX_LAUNCHED("copy_kernel");
-        st = f.solve(wk, wk2, x);
-        if (st) return st;
-    }
-
-    // 4. Preconditioned CG on the exact FP64 operator A = S J^T W J S + alpha I
-    // (starting from x), stopping when the data-space update
-    // |a| |J S p| falls below rtol |J S x| or after max_iter iterations.
+    // 3-4. x = (factor)^-1 b, refined by preconditioned CG.
     int iters = 0;
     double last_rel = 0.0;
     // A shifted factorisation is only a preconditioner: refine even when
     // the caller asked for a direct solve.
     const int allowed = (tau > 0.0 && max_iter == 0) ? 200 : max_iter;
-    if (allowed > 0 && !use_svd) {
-        double *t_dat = f.tmp;
-        st = f.apply(x, t_dat, q);  // q = A x
+    if (!use_svd) {
+        st = f.solve_refine(b, x, r, z, pd, q, wk, wk2, allowed, rtol, iters, last_rel);
         if (st) return st;
-        hipLaunchKernelGGL(sub_kernel, dim3(gm), dim3(256), 0, s, m, b, q, r);  // r = b - A x
-        HMX_LAUNCHED("sub_kernel");
-        double ynorm = 0.0;
-        st = f.reduce(0, t_dat, t_dat, nd, 1, &ynorm);  // |J S x|^2
-        if (st) return st;
-        hipLaunchKernelGGL(copy_kernel, dim3(gm), dim3(256), 0, s, m, r, wk);
-        HMX_LAUNCHED("copy_kernel");
-        st = f.solve(wk, wk2, z);
-        if (st) return st;
-        hipLaunchKernelGGL(copy_kernel, dim3(gm), dim3(256), 0, s, m, z, pd);
-        HMX_LAUNCHED("copy_kernel");
-        double rz = 0.0;
-        st = f.reduce(0, r, z, m, 1, &rz);
-        if (st) return st;
-        for (iters = 1; iters <= allowed; ++iters) {
-            st = f.apply(pd, t_dat, q);  // q = A p, t = J S p
-            if (st) return st;
-            double pq = 0.0, tt = 0.0;
-            st = f.reduce(0, pd, q, m, 1, &pq);
-            if (st) return st;
-            st = f.reduce(0, t_dat, t_dat, nd, 1, &tt);
-            if (st) return st;
-            if (!(pq > 0.0)) break;
-            const double a = rz / pq;
-            hipLaunchKernelGGL(axpy_kernel, dim3(gm), dim3(256), 0, s, m, a, pd, x);
-            HMX_LAUNCHED("axpy_kernel");
-            hipLaunchKernelGGL(axpy_kernel, dim3(gm), dim3(256), 0, s, m, -a, q, r);
-            HMX_LAUNCHED("axpy_kernel");
-            last_rel = std::fabs(a) * std::sqrt(tt) / std::sqrt(ynorm > 0 ? ynorm : 1.0);
-            if (last_rel <= rtol) break;
-            hipLaunchKernelGGL(copy_kernel, dim3(gm), dim3(256), 0, s, m, r, wk);
-            HMX_LAUNCHED("copy_kernel");
-            st = f.solve(wk, wk2, z);
-            if (st) return st;
-            double rz_new = 0.0;
-            st = f.reduce(0, r, z, m, 1, &rz_new);
-            if (st) return st;
-            hipLaunchKernelGGL(xpby_kernel, dim3(gm), dim3(256), 0, s, m, z, rz_new / rz, pd);
-            HMX_LAUNCHED("xpby_kernel");
-            rz = rz_new;
-        }
-        if (iters > allowed) iters = allowed;
         // an unregularised fit whose refinement did not converge
-        if (lstsq && svd_ok && last_rel > rtol) use_svd = true;
+        if (allowed > 0 && lstsq && svd_ok && last_rel > rtol) use_svd = true;
     }
 
     // 4b. The SVD route on the scaled, weighted jacobian X = W^1/2 J S:

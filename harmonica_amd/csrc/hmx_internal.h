@@ -79,6 +79,29 @@ int svd_lsq(double *A, int64_t L, int64_t k, int transposed, const double *y, do
             int filter, double cutoff, double *W, double *vec, int *counters, double *coef,
             int max_sweeps, int *sweeps, int *rank, hipStream_t s);
 constexpr int kSvdMaxSweeps = 60;
+
+// One window of the gradient-boosted fit, gathered on the device: its data
+// (w may be NULL) and its sources.
+struct EqsWindow {
+    const double *e, *n, *u, *w;
+    int64_t nd;
+    const double *pe, *pn, *pu;
+    int64_t m;
+};
+// Bytes of the buffers of a batch of `batch` windows of at most n_max data
+// and m_max sources (FP64 mode, damping > 0; eqs_fit.hip).
+size_t eqs_batch_bytes(int64_t n_max, int64_t m_max, int64_t batch);
+// Forms and factorises the normal matrices of a batch of windows; status
+// (host, batch ints) receives 0 or the 1-based column of a failed pivot.
+int eqs_factor_batch(const EqsWindow *wins, int64_t batch, int64_t n_max, int64_t m_max,
+                     double damping, void *buffer, size_t buffer_bytes, int *status,
+                     hipStream_t s);
+// The coefficients of window b of a factorised batch for the data `data`
+// (the window's residue): right-hand side, solve and CG refinement as
+// hmx_eqs_fit; info (host, 11 doubles) as hmx_eqs_fit's.
+int eqs_solve_factored(const EqsWindow &w, const double *data, double damping, void *buffer,
+                       int64_t b, int64_t batch, int64_t n_max, int64_t m_max, int max_iter,
+                       double rtol, double *coefs, double *info, hipStream_t s);
 // Largest fits that take the SVD route when the Cholesky route cannot match
 // the reference's solver: at most this many sources and matrix entries.
 constexpr int64_t kSvdMaxSrc = 8192;

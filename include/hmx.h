@@ -401,10 +401,14 @@ int hmx_lsq_svd(double *G, int64_t n, int64_t m, const double *y, double damping
                 size_t workspace_bytes, void *stream);
 
 /* Bytes of workspace needed by hmx_eqs_gb_fit for n_data data and windows
- * of at most max_window_data data and max_window_sources sources. */
+ * of at most max_window_data data and max_window_sources sources, with the
+ * same precision_mode, damping and batch as the fit (batch > 1 adds the
+ * buffers of `batch` windows' scaled jacobians, normal matrices and
+ * factors, about 8 (max_window_data + max_window_sources) x
+ * max_window_sources bytes per window). */
 int hmx_eqs_gb_fit_workspace_bytes(int64_t n_data, int64_t max_window_data,
-                                   int64_t max_window_sources, int precision_mode,
-                                   size_t *bytes);
+                                   int64_t max_window_sources, int precision_mode, double damping,
+                                   int64_t batch, size_t *bytes);
 
 /* Replaces EquivalentSourcesGB._gradient_boosting
  * (src/harmonica/_equivalent_sources/gradient_boosted.py:231-283): for each
@@ -427,6 +431,13 @@ int hmx_eqs_gb_fit_workspace_bytes(int64_t n_data, int64_t max_window_data,
  *     windows solved by the SVD route, their total Jacobi sweeps, their
  *     total kept singular values, and the phase seconds of hmx_eqs_fit
  *     summed over the windows].
+ *   batch: in the FP64 mode with damping > 0 and batch > 1, the windows'
+ *     normal matrices (which do not depend on the residue) are formed and
+ *     factorised `batch` windows at a time, every launch covering the batch,
+ *     and each window is then solved from its factor (right-hand side,
+ *     triangular solves, CG refinement as hmx_eqs_fit); a window whose
+ *     factorisation fails goes through hmx_eqs_fit.  Otherwise each window
+ *     is one hmx_eqs_fit call.
  * Indices outside [0, n_data) or [0, n_src) give NaN, never an access out of
  * bounds.  Synchronous. */
 int hmx_eqs_gb_fit(const double *easting, const double *northing, const double *upward,
@@ -434,8 +445,8 @@ int hmx_eqs_gb_fit(const double *easting, const double *northing, const double *
                    const double *p_easting, const double *p_northing, const double *p_upward,
                    int64_t n_src, const int64_t *data_index, const int64_t *data_offset,
                    const int64_t *source_index, const int64_t *source_offset, int64_t n_windows,
-                   double damping, int precision_mode, int max_iter, double rtol, double *coefs,
-                   double *residue, double *rmse, double *info, void *workspace,
+                   double damping, int precision_mode, int64_t batch, int max_iter, double rtol,
+                   double *coefs, double *residue, double *rmse, double *info, void *workspace,
                    size_t workspace_bytes, void *stream);
 
 /* Building blocks of the fit, exposed for testing and reuse (no

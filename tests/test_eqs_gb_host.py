@@ -152,7 +152,7 @@ def test_windows_cover_every_point_and_random_state():
 
 def test_gb_fit_argument_checks_before_device_use():
     L = _lib.lib()
-    info = (ctypes.c_double * 5)()
+    info = (ctypes.c_double * 11)()
     off_bad = np.array([1, 3], dtype=np.int64)
     off_empty = np.array([0, 0], dtype=np.int64)
     i64p = ctypes.POINTER(ctypes.c_int64)
@@ -160,7 +160,7 @@ def test_gb_fit_argument_checks_before_device_use():
     for offsets, msg in ((off_bad, b"start at 0"), (off_empty, b"is empty")):
         st = L.hmx_eqs_gb_fit(dummy, dummy, dummy, 10, dummy, None, dummy, dummy, dummy, 10,
                               dummy, offsets.ctypes.data_as(i64p), dummy,
-                              offsets.ctypes.data_as(i64p), 1, 0.0, 0, 10, 1e-12, dummy, dummy,
-                              dummy, info, dummy, 0, None)
+                              offsets.ctypes.data_as(i64p), 1, 0.0, 0, 1, 10, 1e-12, dummy,
+                              dummy, dummy, info, dummy, 0, None)
         assert st == -1
         assert msg in L.hmx_last_error()

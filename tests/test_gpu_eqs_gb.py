@@ -172,3 +172,39 @@ def test_grid_arguments(hm, model):
         eqs.grid(coordinates=grid, upward=2e3)
     with pytest.warns(FutureWarning):
         eqs.grid(coordinates=grid, bla="bla")
+
+
+@pytest.mark.parametrize("weighted", [False, True])
+def test_batched_factorisation_matches_window_by_window(hm, weighted):
+    """FP64 windows with damping: the normal matrices of batches of windows
+    formed and factorised together (padded to a common size), each window
+    then solved from its factor, against the window-by-window fit
+    (hmx_eqs_fit per window): the padding adds exact zeros, so the
+    coefficients agree to rounding.  12,000 scattered data, windows of
+    about 1500 data, batches of 5 windows (the last partial)."""
+    from harmonica_amd import _eqs_gb
+
+    rng = np.random.default_rng(31)
+    n = 12000
+    coords = (rng.uniform(0, 3e4, n), rng.uniform(0, 3e4, n), rng.uniform(100, 600, n))
+    prisms = np.array([[8e3, 14e3, 5e3, 20e3, -5e3, -2e3]])
+    data = oracle.prism_gravity(coords, prisms, [250.0], "g_z") + rng.normal(0, 0.05, n)
+    w = rng.uniform(0.5, 2.0, n) if weighted else None
+    fits = {}
+    try:
+        for batch in (1, 5):
+            _eqs_gb.FACTOR_BATCH = batch
+            eqs = hm.EquivalentSourcesGB(damping=1e-2, depth=2e3, window_size=8e3,
+                                         random_state=2)
+            eqs.fit(coords, data, weights=w)
+            fits[batch] = eqs
+    finally:
+        _eqs_gb.FACTOR_BATCH = None
+    one, many = fits[1], fits[5]
+    assert one.fit_info_["factor_batch"] == 1 and many.fit_info_["factor_batch"] == 5
+    assert many.fit_info_["windows"] > 5 and many.fit_info_["windows"] % 5 != 0
+    npt.assert_allclose(many.coefs_, one.coefs_, rtol=0, atol=1e-9 * np.abs(one.coefs_).max())
+    npt.assert_allclose(many.rmse_per_iteration_, one.rmse_per_iteration_, rtol=1e-9)
+    grid = grid_coordinates((0, 3e4, 0, 3e4), shape=(20, 20), extra=500.0)
+    p1, p5 = one.predict(grid), many.predict(grid)
+    npt.assert_allclose(p5, p1, rtol=0, atol=1e-10 * np.abs(p1).max())
