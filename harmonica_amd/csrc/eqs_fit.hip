@@ -1772,13 +1772,15 @@ Fit batch_fit(const EqsWindow &w, double damping, const BatchLayout &L, char *bu
 
 int eqs_factor_batch(const EqsWindow *wins, int64_t batch, int64_t n_max, int64_t m_max,
                      double damping, void *buffer, size_t buffer_bytes, int *status,
-                     hipStream_t s) {
+                     double *seconds, hipStream_t s) {
     if (batch <= 0) return HMX_OK;
     if (!(damping > 0.0)) return fail(HMX_ERR_ARG, "batched factorisation needs damping > 0");
     const BatchLayout L = batch_layout(n_max, m_max, batch);
     if (!buffer || buffer_bytes < L.total)
         return fail(HMX_ERR_ARG, "batch buffer too small: %zu < %zu bytes", buffer_bytes, L.total);
     char *buf = static_cast<char *>(buffer);
+    PhaseEvents ev;
+    ev.record(0, s);
     double *X = reinterpret_cast<double *>(buf + L.x_off);
     const int64_t bs_x = L.n_max * L.m_pad, bs_a = L.m_pad * L.m_pad;
     const int64_t bs_inv = L.nblk * kNB * kNB;
@@ -1804,11 +1806,17 @@ int eqs_factor_batch(const EqsWindow *wins, int64_t batch, int64_t n_max, int64_
     int st = launch_syrk<double>(X, L.m_pad, L.n_max, L.m_pad, A, L.m_pad, 0.0, 1.0, damping, s,
                                  (int)batch, bs_x, bs_a);
     if (st) return st;
+    ev.record(1, s);
     Fit f = batch_fit(wins[0], damping, L, buf, 0, s);
     f.m = L.m_pad;
     f.DinvT = f.Dinv + batch * bs_inv;
-    return f.cholesky(status, (int)batch, bs_a, bs_inv,
-                      reinterpret_cast<int *>(buf + L.infos_off));
+    st = f.cholesky(status, (int)batch, bs_a, bs_inv, reinterpret_cast<int *>(buf + L.infos_off));
+    if (st) return st;
+    ev.record(2, s);
+    HMX_HIP(hipStreamSynchronize(s));
+    seconds[0] = ev.seconds(0, 1);
+    seconds[1] = ev.seconds(1, 2);
+    return HMX_OK;
 }
 
 int eqs_solve_factored(const EqsWindow &w, const double *data, double damping, void *buffer,
@@ -1822,6 +1830,8 @@ int eqs_solve_factored(const EqsWindow &w, const double *data, double damping, v
     const int64_t mp = L.m_pad;
     double *bv = vec, *x = vec + mp, *r = x + mp, *z = r + mp, *pd = z + mp, *q = pd + mp,
            *wk = q + mp, *wk2 = wk + mp;
+    PhaseEvents ev;
+    ev.record(0, s);
     int st = f.jt(w.w, data, 0.0, nullptr, bv);  // b = S J^T W d
     if (st) return st;
     int iters = 0;
@@ -1830,11 +1840,13 @@ int eqs_solve_factored(const EqsWindow &w, const double *data, double damping, v
     if (st) return st;
     hipLaunchKernelGGL(div_kernel, dim3(blocks_for(w.m)), dim3(256), 0, s, w.m, x, f.scale, coefs);
     HMX_LAUNCHED("div_kernel");
+    ev.record(1, s);
     st = f.check_solves();
     if (st) return st;
     for (int k = 0; k < 11; ++k) info[k] = 0.0;
     info[0] = (double)iters;
     info[1] = last_rel;
+    info[10] = ev.seconds(0, 1);
     return HMX_OK;
 }
 

@@ -327,9 +327,12 @@ int hmx_eqs_gb_fit(const double *easting, const double *northing, const double *
                                             slot_array(b, 5), slot_array(b, 6), slot_array(b, 7),
                                             ns};
             }
+            double secs[2] = {0.0, 0.0};
             st = eqs_factor_batch(wins.data(), nbat, max_wd, max_ws, damping, batch_buf,
-                                  L.batch_bytes, status.data(), s);
+                                  L.batch_bytes, status.data(), secs, s);
             if (st) return st;
+            phase[0] += secs[0];
+            phase[1] += secs[1];
         }
         const int64_t wend = B > 1 ? w0 + nbat : w0 + 1;
         for (int64_t w = w0; w < wend; ++w) {

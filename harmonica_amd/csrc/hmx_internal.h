@@ -92,10 +92,11 @@ struct EqsWindow {
 // and m_max sources (FP64 mode, damping > 0; eqs_fit.hip).
 size_t eqs_batch_bytes(int64_t n_max, int64_t m_max, int64_t batch);
 // Forms and factorises the normal matrices of a batch of windows; status
-// (host, batch ints) receives 0 or the 1-based column of a failed pivot.
+// (host, batch ints) receives 0 or the 1-based column of a failed pivot,
+// seconds (host, 2) the times of the normal matrices and of the factors.
 int eqs_factor_batch(const EqsWindow *wins, int64_t batch, int64_t n_max, int64_t m_max,
                      double damping, void *buffer, size_t buffer_bytes, int *status,
-                     hipStream_t s);
+                     double *seconds, hipStream_t s);
 // The coefficients of window b of a factorised batch for the data `data`
 // (the window's residue): right-hand side, solve and CG refinement as
 // hmx_eqs_fit; info (host, 11 doubles) as hmx_eqs_fit's.
