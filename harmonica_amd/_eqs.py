@@ -277,7 +277,7 @@ def fit_device(coords, data, weights, points, damping, precision="float64", max_
 #: Keys of the solver statistics of hmx_eqs_fit (``fit_info_``): conjugate
 #: gradient iterations, their last relative update, the diagonal shift and
 #: factorisation retries (fits larger than the SVD route), the largest
-#: diagonal entry of the normal matrix, whether the SVD route solved the fit
+#: diagonal entry of the normal matrix (when a shift was needed, else 0), whether the SVD route solved the fit
 #: (1.0), its Jacobi sweeps and kept singular values, and the seconds spent
 #: forming the normal matrix, factorising it, and solving and refining.
 FIT_INFO_KEYS = ("iterations", "last_relative_update", "shift", "retries", "max_diagonal",

@@ -360,7 +360,8 @@ int hmx_eqs_fit_workspace_bytes(int64_t n_data, int64_t n_src, int precision_mod
  *     data-space prediction per iteration);
  *   coefs: (n_src) device output;
  *   info: HOST array of 11 doubles: [CG iterations, last relative update,
- *     shift tau, factorisation retries, max diagonal of the normal matrix,
+ *     shift tau, factorisation retries, max diagonal of the normal matrix
+ *     (set when a shift was needed, else 0),
  *     1 if the SVD route solved the fit, its Jacobi sweeps (negative if
  *     the sweep limit stopped them), its kept singular values, seconds of
  *     the normal matrix (column statistics and jacobian included), of the
@@ -427,7 +428,8 @@ int hmx_eqs_gb_fit_workspace_bytes(int64_t n_data, int64_t max_window_data,
  *     before the first window and after each (rmse_per_iteration_, :250,
  *     :278);
  *   info: HOST array of 11 doubles: [total CG iterations, last relative
- *     update, largest shift, total factorisation retries, largest diagonal,
+ *     update, largest shift, total factorisation retries, largest diagonal
+ *     (of the windows that needed a shift),
  *     windows solved by the SVD route, their total Jacobi sweeps, their
  *     total kept singular values, and the phase seconds of hmx_eqs_fit
  *     summed over the windows].
