@@ -1394,10 +1394,10 @@ struct PhaseEvents {
     }
     ~PhaseEvents() {
         for (auto &x : e)
-            if (x) hipEventDestroy(x);
+            if (x) (void)hipEventDestroy(x);
     }
     void record(int k, hipStream_t s) {
-        if (e[k]) hipEventRecord(e[k], s);
+        if (e[k]) (void)hipEventRecord(e[k], s);
     }
     double seconds(int a, int b) {
         float ms = 0.0f;
