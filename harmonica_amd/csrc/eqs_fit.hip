@@ -1487,6 +1487,21 @@ struct Fit {
     int64_t mf = 0;
     int64_t fm() const { return mf ? mf : m; }
 
+    // The second stream of the factorisation's lookahead and its two
+    // events; the stream is drained before it is destroyed.
+    struct SideStream {
+        hipStream_t s2 = nullptr;
+        hipEvent_t steps = nullptr, rest = nullptr;
+        ~SideStream() {
+            if (s2) {
+                (void)hipStreamSynchronize(s2);
+                (void)hipStreamDestroy(s2);
+            }
+            if (steps) (void)hipEventDestroy(steps);
+            if (rest) (void)hipEventDestroy(rest);
+        }
+    };
+
     // Blocked right-looking factorisation A = U^T U (upper, in place) in two
     // levels: within an outer block of kOuter rows, kNB-row steps factor the
     // diagonal block, solve their row panel up to column m and update only
@@ -1497,6 +1512,14 @@ struct Fit {
     // diagonal-block inverses), every launch covering the whole batch;
     // pivot then receives one status per matrix (0 or the 1-based column of
     // its failed pivot, from the device array infos).
+    // Lookahead (one matrix with more than two outer blocks left): the
+    // rank-kOuter update is split into the next outer block's rows, updated
+    // on the fit's stream, and the rows past them, updated on a second
+    // stream while the next outer block's kNB-row steps (one-workgroup
+    // diagonal factorisations and narrow panel products, which leave most
+    // of the device idle) run on the fit's stream.  Rows [P + kOuter, n)
+    // written by the side update of outer block P are next touched by the
+    // update of outer block P + kOuter, which waits for it.
     int cholesky(int *pivot, int batch = 1, int64_t bs_a = 0, int64_t bs_inv = 0,
                  int *infos = nullptr) {
         const int64_t n = fm();
@@ -1504,6 +1527,20 @@ struct Fit {
         hipLaunchKernelGGL(fill_int_kernel, dim3(blocks_for(batch)), dim3(256), 0, s, dev_info,
                            (int64_t)batch, 0);
         HMX_LAUNCHED("fill_int_kernel");
+        SideStream side;
+        const bool ahead = batch == 1 && n > 2 * (int64_t)kOuter;
+        if (ahead) {
+            HMX_HIP(hipStreamCreateWithFlags(&side.s2, hipStreamNonBlocking));
+            HMX_HIP(hipEventCreateWithFlags(&side.steps, hipEventDisableTiming));
+            HMX_HIP(hipEventCreateWithFlags(&side.rest, hipEventDisableTiming));
+        }
+        bool rest_pending = false;
+        auto join = [&]() -> int {  // the fit's stream waits for the last side update
+            if (!rest_pending) return HMX_OK;
+            HMX_HIP(hipStreamWaitEvent(s, side.rest, 0));
+            rest_pending = false;
+            return HMX_OK;
+        };
         for (int64_t P = 0; P < n; P += kOuter) {
             const int64_t Pend = P + kOuter < n ? P + kOuter : n;
             for (int64_t p = P; p < Pend; p += kNB) {
@@ -1538,11 +1575,38 @@ struct Fit {
             }
             if (Pend < n) {
                 const int64_t K = Pend - P, rest = n - Pend;
-                int st = launch_syrk<double>(A + P * n + Pend, n, K, rest, A + Pend * n + Pend, n,
-                                             1.0, -1.0, 0.0, s, batch, bs_a, bs_a);
-                if (st) return st;
+                const double *panel = A + P * n + Pend;  // rows [P, Pend), columns [Pend, n)
+                double *T = A + Pend * n + Pend;
+                int st = HMX_OK;
+                if (!ahead || rest <= kOuter) {
+                    st = join();
+                    if (st) return st;
+                    st = launch_syrk<double>(panel, n, K, rest, T, n, 1.0, -1.0, 0.0, s, batch,
+                                             bs_a, bs_a);
+                    if (st) return st;
+                } else {
+                    // rows past the next outer block, on the side stream once
+                    // this outer block's rows are final
+                    HMX_HIP(hipEventRecord(side.steps, s));
+                    HMX_HIP(hipStreamWaitEvent(side.s2, side.steps, 0));
+                    st = join();  // the previous side update wrote the next outer block's rows
+                    if (st) return st;
+                    const int64_t top = kOuter, far = rest - kOuter;
+                    st = launch_syrk<double>(panel + top, n, K, far, T + top * n + top, n, 1.0,
+                                             -1.0, 0.0, side.s2);
+                    if (st) return st;
+                    HMX_HIP(hipEventRecord(side.rest, side.s2));
+                    rest_pending = true;
+                    // the next outer block's rows, all columns from Pend (the
+                    // entries below the diagonal it covers are never read)
+                    st = launch_gemm<double>(panel, n, panel, n, K, top, rest, T, n, 1.0, -1.0,
+                                             0.0, false, s);
+                    if (st) return st;
+                }
             }
         }
+        int st = join();
+        if (st) return st;
         HMX_HIP(hipMemcpyAsync(pivot, dev_info, (size_t)batch * sizeof(int), hipMemcpyDeviceToHost,
                                s));
         HMX_HIP(hipStreamSynchronize(s));
