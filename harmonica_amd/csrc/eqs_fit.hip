@@ -635,73 +635,101 @@ __global__ __launch_bounds__(1024) void potrf_diag_kernel(double *__restrict__ D
 // (U^T y = r) owns block b = w of y; workgroup w of the backward solve
 // (U x = y) owns block b = nb - 1 - w of x.  A block needs the solved
 // blocks before it (forward) or after it (backward), which belong to
-// workgroups of lower index: each wavefront waits for a block's flag
-// (flags[k] == epoch, polled with agent-scope loads in every lane), reads the
-// block's solution with agent-scope loads, and accumulates its product with
-// the block of U it prefetched into registers before waiting.  The
-// dependencies are visited oldest first, so the wait that matters (the
-// block just before) comes last.  The block's own triangle goes through its
-// stored inverse, then the solution is written and the flag released.
+// workgroups of lower index.  The output vector is filled with a sentinel
+// before the launch (kSolveSentinel, a signalling-NaN pattern that no
+// arithmetic produces: NaN results are quiet NaNs), every solved entry is
+// written with an agent-scope store, and a consumer lane polls the entries
+// it needs with agent-scope loads until none is the sentinel: the poll
+// returns the values themselves, so a dependency costs one round trip to
+// the device-coherent level instead of a flag and then a load.  Each
+// workgroup accumulates the product of a dependency with the block of U it
+// prefetched into registers before polling; the dependencies are visited
+// oldest first, so the wait that matters (the block just before) comes
+// last, and the block's own triangle goes through its stored inverse.
 // Workgroups of a launch are dispatched in index order on each XCD, so the
 // lowest unfinished workgroup always has its dependencies done and the
 // solve makes progress; every wait is still bounded (kTrsvSpinLimit polls)
-// and a timed-out wait sets *err and goes on, so every wavefront ends.
+// and a timed-out wait sets *err and goes on (with 0 for the value), so
+// every wavefront ends.
 constexpr int kTrsvSpinLimit = 1 << 22;
-
-// Every lane polls (one request per wavefront: same address) with
-// agent-scope loads, which see the device-wide value without invalidating
-// the caches; all lanes read the same value and leave the loop together.
-// The workgroup-scope acquire fence after the loop keeps the block's loads
-// after it; those are agent-scope loads too (load_agent), so they see what
-// the writer released before its flag.
-__device__ __forceinline__ void wait_flag(const int *flag, int epoch, int *err) {
-    int spins = 0;
-    while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
-        if (++spins > kTrsvSpinLimit) {
-            if ((threadIdx.x & 63) == 0) atomicOr(err, 1);
-            break;
-        }
-        __builtin_amdgcn_s_sleep(1);
-    }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-}
+constexpr long long kSolveSentinel = 0x7ff4dead5eed0001LL;
 
 __device__ __forceinline__ double load_agent(const double *p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+__device__ __forceinline__ void store_agent(double *p, double v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ bool is_sentinel(double v) {
+    return __double_as_longlong(v) == kSolveSentinel;
+}
+
+// Starts the copy of a kNB x kNB block (row-major, 128 KB) into LDS
+// straight from global memory (16 bytes per lane, 1 KB runs per wave); the
+// copy is complete once every wave has waited for its vector memory
+// operations and the workgroup has met at a barrier.  1024 threads.
+__device__ __forceinline__ void block_to_lds(const double *__restrict__ src, double *dst) {
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int run = 8 * w + i;  // 128 runs of 128 doubles
+        __builtin_amdgcn_global_load_lds((const void *)(src + run * kNB + 2 * l),
+                                         (__attribute__((address_space(3))) void *)(dst + run * kNB),
+                                         16, 0, 0);
+    }
+}
+
+// The solved entry *p, polled until it is not the sentinel.
+__device__ __forceinline__ double wait_value(const double *p, int *err) {
+    double v = load_agent(p);
+    int spins = 0;
+    while (is_sentinel(v)) {
+        if (++spins > kTrsvSpinLimit) {
+            atomicOr(err, 1);
+            return 0.0;
+        }
+        __builtin_amdgcn_s_sleep(1);
+        v = load_agent(p);
+    }
+    return v;
+}
+
 // Forward: y_b = Dinv_b^T (r_b - sum_{k < b} U_kb^T y_k), U_kb = rows of
 // block k, columns of block b.  Thread (g, t) = (tid / 128, tid % 128)
-// takes column t of the block and rows 16 g .. 16 g + 15 of every block k.
+// takes column t of the block and rows 16 g .. 16 g + 15 of every block k
+// (lanes 0-15 of its wavefront poll those 16 entries of y_k).
 __global__ __launch_bounds__(1024) void trsv_fwd_sync_kernel(
     const double *__restrict__ U, int64_t ld, int64_t m, const double *__restrict__ Dinv,
-    const double *__restrict__ r, double *__restrict__ y, int *__restrict__ flags, int epoch,
-    int *__restrict__ err) {
+    const double *__restrict__ r, double *__restrict__ y, int *__restrict__ err) {
+    __shared__ double dinv[kNB * kNB];  // Dinv_b, copied during the dependency loop
     __shared__ double part[8][kNB];
     __shared__ double z[kNB];
     const int b = blockIdx.x;
     const int t = threadIdx.x & (kNB - 1), g = threadIdx.x >> 7, lane = threadIdx.x & 63;
     const int64_t c0 = (int64_t)b * kNB, c = c0 + t;
     const bool cv = c < m;
-    double mv[16];  // Dinv_b[16 g + q][t]
-#pragma unroll
-    for (int q = 0; q < 16; ++q) mv[q] = Dinv[(int64_t)b * kNB * kNB + (16 * g + q) * kNB + t];
+    block_to_lds(Dinv + (int64_t)b * kNB * kNB, dinv);
     double u[16];
+    // Columns past m (last block only) read column m - 1: finite values
+    // whose sums are dropped (z = 0 there).
     auto load_u = [&](int k) {
-        const double *col = U + ((int64_t)k * kNB + 16 * g) * ld + c;
+        const double *col = U + ((int64_t)k * kNB + 16 * g) * ld + (cv ? c : m - 1);
 #pragma unroll
-        for (int q = 0; q < 16; ++q) u[q] = cv ? col[q * ld] : 0.0;
+        for (int q = 0; q < 16; ++q) u[q] = col[q * ld];
     };
     double acc = 0.0;
     if (b > 0) load_u(0);
     for (int k = 0; k < b; ++k) {
-        wait_flag(flags + k, epoch, err);
-        const double yv = lane < 16 ? load_agent(y + (int64_t)k * kNB + 16 * g + lane) : 0.0;
+        // blocks before b are whole (only the last block may be partial)
+        const double yv = lane < 16 ? wait_value(y + (int64_t)k * kNB + 16 * g + lane, err) : 0.0;
 #pragma unroll
         for (int q = 0; q < 16; ++q) acc = fma(u[q], __shfl(yv, q, 64), acc);
         if (k + 1 < b) load_u(k + 1);
     }
     part[g][t] = acc;
+    __builtin_amdgcn_s_waitcnt(0);  // this wave's part of the Dinv_b copy
     __syncthreads();
     if (g == 0) {
         double s = 0.0;
@@ -714,7 +742,7 @@ __global__ __launch_bounds__(1024) void trsv_fwd_sync_kernel(
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
         const int l = 16 * g + q;
-        d = l <= t ? fma(mv[q], z[l], d) : d;
+        d = l <= t ? fma(dinv[l * kNB + t], z[l], d) : d;
     }
     part[g][t] = d;
     __syncthreads();
@@ -722,21 +750,18 @@ __global__ __launch_bounds__(1024) void trsv_fwd_sync_kernel(
         double s = 0.0;
 #pragma unroll
         for (int q = 0; q < 8; ++q) s += part[q][t];
-        y[c] = s;
+        store_agent(y + c, s);
     }
-    __syncthreads();
-    if (threadIdx.x == 0)
-        __hip_atomic_store(flags + b, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Backward: x_b = Dinv_b (y_b - sum_{k > b} U_bk x_k).  Wavefront v of the
 // workgroup takes rows 8 v .. 8 v + 7 of the block, lane l columns l and
 // l + 64 of every block k (partial sums per lane, added across the lanes
-// once at the end).
+// once at the end); lane l polls entries l and l + 64 of x_k.
 __global__ __launch_bounds__(1024) void trsv_bwd_sync_kernel(
     const double *__restrict__ U, int64_t ld, int64_t m, const double *__restrict__ DinvT,
-    const double *__restrict__ y, double *__restrict__ x, int *__restrict__ flags, int epoch,
-    int *__restrict__ err) {
+    const double *__restrict__ y, double *__restrict__ x, int *__restrict__ err) {
+    __shared__ double dinvt[kNB * kNB];  // DinvT_b, copied during the dependency loop
     __shared__ double part[8][kNB];
     __shared__ double z[kNB];
     const int nb = (int)((m + kNB - 1) / kNB);
@@ -744,18 +769,20 @@ __global__ __launch_bounds__(1024) void trsv_bwd_sync_kernel(
     const int lane = threadIdx.x & 63, v = threadIdx.x >> 6;
     const int t = threadIdx.x & (kNB - 1), g = threadIdx.x >> 7;
     const int64_t r0 = (int64_t)b * kNB;
-    double mv[16];  // DinvT_b[16 g + q][t] = Dinv_b[t][16 g + q]
-#pragma unroll
-    for (int q = 0; q < 16; ++q) mv[q] = DinvT[(int64_t)b * kNB * kNB + (16 * g + q) * kNB + t];
+    block_to_lds(DinvT + (int64_t)b * kNB * kNB, dinvt);
     double u[16];
+    // Rows and columns past m read row or column m - 1: finite values that
+    // meet x = 0 (columns) or whose sums are dropped (rows, z = 0 there).
     auto load_u = [&](int k) {
         const int64_t cb = (int64_t)k * kNB;
+        const int64_t c0 = cb + lane < m ? cb + lane : m - 1;
+        const int64_t c1 = cb + lane + 64 < m ? cb + lane + 64 : m - 1;
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
             const int64_t row = r0 + 8 * v + i;
-            const double *p = U + (row < m ? row : m - 1) * ld + cb;
-            u[2 * i] = (row < m && cb + lane < m) ? p[lane] : 0.0;
-            u[2 * i + 1] = (row < m && cb + lane + 64 < m) ? p[lane + 64] : 0.0;
+            const double *p = U + (row < m ? row : m - 1) * ld;
+            u[2 * i] = p[c0];
+            u[2 * i + 1] = p[c1];
         }
     };
     double acc[8];
@@ -763,10 +790,10 @@ __global__ __launch_bounds__(1024) void trsv_bwd_sync_kernel(
     for (int i = 0; i < 8; ++i) acc[i] = 0.0;
     if (b + 1 < nb) load_u(nb - 1);
     for (int k = nb - 1; k > b; --k) {
-        wait_flag(flags + k, epoch, err);
+        // entries past m are never written: not polled
         const int64_t cb = (int64_t)k * kNB;
-        const double x0 = cb + lane < m ? load_agent(x + cb + lane) : 0.0;
-        const double x1 = cb + lane + 64 < m ? load_agent(x + cb + lane + 64) : 0.0;
+        const double x0 = cb + lane < m ? wait_value(x + cb + lane, err) : 0.0;
+        const double x1 = cb + lane + 64 < m ? wait_value(x + cb + lane + 64, err) : 0.0;
 #pragma unroll
         for (int i = 0; i < 8; ++i) acc[i] = fma(u[2 * i + 1], x1, fma(u[2 * i], x0, acc[i]));
         if (k - 1 > b) load_u(k - 1);
@@ -782,12 +809,13 @@ __global__ __launch_bounds__(1024) void trsv_bwd_sync_kernel(
         const int64_t row = r0 + 8 * v + lane;
         z[8 * v + lane] = row < m ? y[row] - s : 0.0;
     }
+    __builtin_amdgcn_s_waitcnt(0);  // this wave's part of the DinvT_b copy
     __syncthreads();
-    double d = 0.0;  // (Dinv_b z)[t] = sum_{l >= t} Dinv_b[t][l] z[l]
+    double d = 0.0;  // (Dinv_b z)[t] = sum_{l >= t} DinvT_b[l][t] z[l]
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
         const int l = 16 * g + q;
-        d = l >= t ? fma(mv[q], z[l], d) : d;
+        d = l >= t ? fma(dinvt[l * kNB + t], z[l], d) : d;
     }
     part[g][t] = d;
     __syncthreads();
@@ -795,11 +823,14 @@ __global__ __launch_bounds__(1024) void trsv_bwd_sync_kernel(
         double s = 0.0;
 #pragma unroll
         for (int q = 0; q < 8; ++q) s += part[q][t];
-        x[r0 + t] = s;
+        store_agent(x + r0 + t, s);
     }
-    __syncthreads();
-    if (threadIdx.x == 0)
-        __hip_atomic_store(flags + b, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// p[0, n) = the solves' sentinel.
+__global__ void fill_sentinel_kernel(double *__restrict__ p, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = __longlong_as_double(kSolveSentinel);
 }
 
 __global__ void fill_int_kernel(int *__restrict__ p, int64_t n, int v) {
@@ -1349,10 +1380,8 @@ inline unsigned blocks_for(int64_t n, int per = 256) {
     return (unsigned)(b < 1 ? 1 : b);
 }
 
-// Bytes of the substitutions' dependency flags (2 per block) and error word.
-inline size_t solve_flag_bytes(int64_t m) {
-    return (size_t)(2 * ((m + kNB - 1) / kNB) + 1) * sizeof(int);
-}
+// Bytes of the substitutions' error word.
+inline size_t solve_err_bytes() { return sizeof(int); }
 
 // Whether a fit takes the SVD route when its Cholesky route cannot match
 // the reference's solver (the route is sized for these fits only).
@@ -1423,10 +1452,9 @@ struct Fit {
     double *scale, *A, *Dinv, *DinvT, *tmp, *part;
     double host_part[256];
     int *dinfo;
-    // dependency flags of the two substitution launches (2 per block), the
-    // error word of their waits, and the epoch of the last solve
-    int *flags = nullptr, *terr = nullptr;
-    int epoch = 0;
+    // the error word of the substitutions' waits (device), and the same
+    // pointer once a solve has been set up
+    int *err_word = nullptr, *terr = nullptr;
     double *split = nullptr;  // partial sums of the split reductions
 
     // out = (1 / s) J^T W t + alpha p (W = diag(w_), w_ and p may be NULL).
@@ -1613,13 +1641,11 @@ struct Fit {
         return HMX_OK;
     }
 
-    // Flags and error word at zero, before the first solve.
+    // The error word at zero, before the first solve.
     int init_solves() {
-        const int64_t n = 2 * ((fm() + kNB - 1) / kNB) + 1;
-        hipLaunchKernelGGL(fill_int_kernel, dim3(blocks_for(n)), dim3(256), 0, s, flags, n, 0);
+        hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(64), 0, s, err_word, (int64_t)1, 0);
         HMX_LAUNCHED("fill_int_kernel");
-        terr = flags + n - 1;
-        epoch = 0;
+        terr = err_word;
         return HMX_OK;
     }
 
@@ -1628,12 +1654,15 @@ struct Fit {
     int solve(double *r, double *y, double *x) {
         const int64_t n = fm();
         const int nb = (int)((n + kNB - 1) / kNB);
-        ++epoch;
+        hipLaunchKernelGGL(fill_sentinel_kernel, dim3(blocks_for(n)), dim3(256), 0, s, y, n);
+        HMX_LAUNCHED("fill_sentinel_kernel");
         hipLaunchKernelGGL(trsv_fwd_sync_kernel, dim3(nb), dim3(1024), 0, s, A, n, n, Dinv, r, y,
-                           flags, epoch, terr);
+                           terr);
         HMX_LAUNCHED("trsv_fwd_sync_kernel");
+        hipLaunchKernelGGL(fill_sentinel_kernel, dim3(blocks_for(n)), dim3(256), 0, s, x, n);
+        HMX_LAUNCHED("fill_sentinel_kernel");
         hipLaunchKernelGGL(trsv_bwd_sync_kernel, dim3(nb), dim3(1024), 0, s, A, n, n, DinvT, y, x,
-                           flags + nb, epoch, terr);
+                           terr);
         HMX_LAUNCHED("trsv_bwd_sync_kernel");
         return HMX_OK;
     }
@@ -1760,7 +1789,7 @@ __global__ void mfma_probe_kernel(int dtype, const double *__restrict__ a,
 struct BatchLayout {
     int64_t m_pad, n_max, nblk;
     size_t x_off, a_off, dinv_off, scale_off, vec_off, tmp_off, part_off, split_off, dinfo_off,
-        flags_off, infos_off, total;
+        err_off, infos_off, total;
 };
 
 inline size_t al256(size_t b) { return (b + 255) / 256 * 256; }
@@ -1790,8 +1819,8 @@ BatchLayout batch_layout(int64_t n_max, int64_t m_max, int64_t batch) {
     off += al256(split_doubles(n_max > L.m_pad ? n_max : L.m_pad) * sizeof(double));
     L.dinfo_off = off;
     off += 256;
-    L.flags_off = off;
-    off += al256(solve_flag_bytes(L.m_pad));
+    L.err_off = off;
+    off += al256(solve_err_bytes());
     L.infos_off = off;
     off += al256(B * sizeof(int));
     L.total = off;
@@ -1828,7 +1857,7 @@ Fit batch_fit(const EqsWindow &w, double damping, const BatchLayout &L, char *bu
     f.part = reinterpret_cast<double *>(buf + L.part_off);
     f.split = reinterpret_cast<double *>(buf + L.split_off);
     f.dinfo = reinterpret_cast<int *>(buf + L.dinfo_off);
-    f.flags = reinterpret_cast<int *>(buf + L.flags_off);
+    f.err_word = reinterpret_cast<int *>(buf + L.err_off);
     return f;
 }
 
@@ -1955,7 +1984,7 @@ int hmx_cholesky_workspace_bytes(int64_t m, size_t *bytes) {
     if (m <= 0) return fail(HMX_ERR_ARG, "empty matrix");
     const size_t nblk = (size_t)((m + kNB - 1) / kNB);
     *bytes = 2 * nblk * kNB * kNB * sizeof(double) + (2 * (size_t)m + 256) * sizeof(double) + 256 +
-             solve_flag_bytes(m);
+             solve_err_bytes();
     return HMX_OK;
 }
 
@@ -1980,7 +2009,7 @@ int hmx_cholesky_solve(double *A, int64_t m, double *x, int *info, void *workspa
     double *wk2 = wk + m;
     f.part = wk2 + m;
     f.dinfo = reinterpret_cast<int *>(f.part + 256);
-    f.flags = f.dinfo + 64;
+    f.err_word = f.dinfo + 64;
     st = f.cholesky(info);
     if (st) return st;
     if (*info) return HMX_OK;
@@ -2010,7 +2039,7 @@ int hmx_eqs_fit_workspace_bytes(int64_t n_data, int64_t n_src, int precision_mod
              + 2 * nblk * kNB * kNB * sizeof(double)  // inverses of diagonal blocks
              + (10 * m + n + 256) * sizeof(double)  // 10 m-vectors, n-vector, partials
              + split_doubles(m > n ? m : n) * sizeof(double) + 256  // split sums, info
-             + solve_flag_bytes(n_src);                             // substitution flags
+             + solve_err_bytes();                                   // substitutions' error word
     return HMX_OK;
 }
 
@@ -2086,7 +2115,7 @@ int hmx_eqs_fit(const double *easting, const double *northing, const double *upw
     f.part = f.tmp + nd;
     f.split = f.part + 256;
     f.dinfo = reinterpret_cast<int *>(f.split + split_doubles(m > nd ? m : nd));
-    f.flags = f.dinfo + 64;  // after the 256 bytes of info
+    f.err_word = f.dinfo + 64;  // after the 256 bytes of info
     const unsigned gm = blocks_for(m), gy = (unsigned)(nd < 8192 ? nd : 8192);
 
     // phases: 0 start, 1 factorisation start, 2 factorisation end, 3 end
