@@ -696,6 +696,26 @@ __device__ __forceinline__ double wait_value(const double *p, int *err) {
     return v;
 }
 
+// The solved entries *p0 (if v0) and *p1 (if v1), polled together until
+// neither is the sentinel (0 for an entry not wanted).
+__device__ __forceinline__ void wait_pair(const double *p0, bool v0, const double *p1, bool v1,
+                                          int *err, double &x0, double &x1) {
+    x0 = v0 ? load_agent(p0) : 0.0;
+    x1 = v1 ? load_agent(p1) : 0.0;
+    int spins = 0;
+    while (is_sentinel(x0) || is_sentinel(x1)) {
+        if (++spins > kTrsvSpinLimit) {
+            atomicOr(err, 1);
+            x0 = 0.0;
+            x1 = 0.0;
+            return;
+        }
+        __builtin_amdgcn_s_sleep(1);
+        if (is_sentinel(x0)) x0 = load_agent(p0);
+        if (is_sentinel(x1)) x1 = load_agent(p1);
+    }
+}
+
 // Forward: y_b = Dinv_b^T (r_b - sum_{k < b} U_kb^T y_k), U_kb = rows of
 // block k, columns of block b.  Thread (g, t) = (tid / 128, tid % 128)
 // takes column t of the block and rows 16 g .. 16 g + 15 of every block k
@@ -792,8 +812,9 @@ __global__ __launch_bounds__(1024) void trsv_bwd_sync_kernel(
     for (int k = nb - 1; k > b; --k) {
         // entries past m are never written: not polled
         const int64_t cb = (int64_t)k * kNB;
-        const double x0 = cb + lane < m ? wait_value(x + cb + lane, err) : 0.0;
-        const double x1 = cb + lane + 64 < m ? wait_value(x + cb + lane + 64, err) : 0.0;
+        double x0, x1;
+        wait_pair(x + cb + lane, cb + lane < m, x + cb + lane + 64, cb + lane + 64 < m, err, x0,
+                  x1);
 #pragma unroll
         for (int i = 0; i < 8; ++i) acc[i] = fma(u[2 * i + 1], x1, fma(u[2 * i], x0, acc[i]));
         if (k - 1 > b) load_u(k - 1);
