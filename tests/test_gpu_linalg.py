@@ -158,10 +158,12 @@ def test_syrk_upper(env, dtype, m):
     assert np.all(got[il] == 7.0)  # lower triangle untouched
 
 
-@pytest.mark.parametrize("m", [5, 128, 129, 300, 700, 1500, 2100])
+@pytest.mark.parametrize("m", [5, 128, 129, 300, 700, 1500, 2100, 4500])
 def test_cholesky_solve(env, m):
     """Sizes below, across and beyond the 1024-row outer block (the rank-1024
-    trailing update runs for m > 1024)."""
+    trailing update runs for m > 1024; beyond two outer blocks the update of
+    the rows past the next outer block runs on a second stream: once at
+    2100, three times at 4500)."""
     torch, _lib, L = env
     rng = np.random.default_rng(m)
     X = rng.normal(size=(m + 20, m))
@@ -182,8 +184,10 @@ def test_cholesky_solve(env, m):
     npt.assert_allclose(U.T @ U, A, atol=1e-10 * np.abs(A).max())
 
 
-@pytest.mark.parametrize("m,bad", [(200, 150), (1300, 1200)])
+@pytest.mark.parametrize("m,bad", [(200, 150), (1300, 1200), (4500, 3000)])
 def test_cholesky_reports_indefinite(env, m, bad):
+    """The failed pivot's column; at 4500 the factorisation stops early while
+    an update on the second stream may still run (it is waited for)."""
     torch, _lib, L = env
     A = np.eye(m)
     A[bad, bad] = -1.0
