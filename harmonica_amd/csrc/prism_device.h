@@ -205,8 +205,10 @@ __device__ __forceinline__ bool any_lane(bool need) { return __any(need) != 0; }
 // face_angle).  With kTop, *ftop receives the top-face sum
 // sum_{i,j} (-1)^(i+j) A_z(i, j, 0) (the derivative with respect to the top).
 // Without kGeneral the caller guarantees the point lies between neither the
-// two x-planes nor the two y-planes (no mixed quotient, no 2 pi branch).
-template <bool kTop, bool kGeneral, bool kShare = false>
+// two x-planes nor the two y-planes (no mixed quotient, no 2 pi branch);
+// kGenX lifts the first condition (the logarithms along x take their general
+// form; the face angles need the 2 pi branch only between both pairs).
+template <bool kTop, bool kGeneral, bool kShare = false, bool kGenX = kGeneral>
 __device__ __forceinline__ double gu_sum(double x0, double x1, double y0, double y1, double z0,
                                          double z1, const dm::Tables &tab, double *ftop,
                                          int share = 0, double (*rb)[2] = nullptr) {
@@ -261,7 +263,7 @@ __device__ __forceinline__ double gu_sum(double x0, double x1, double y0, double
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
         const double c = j ? y1 : y0, cs = j ? ys1 : ys0;
-        const double l = log_quad<kGeneral>(x0, x1, r[0][j][0], r[0][j][1], r[1][j][0],
+        const double l = log_quad<kGeneral || kGenX>(x0, x1, r[0][j][0], r[0][j][1], r[1][j][0],
                                             r[1][j][1], cs, zt0, zt1, flip_x, tab);
         sum = fma(j ? -c : c, l, sum);
     }
@@ -301,8 +303,10 @@ __device__ __forceinline__ double gu_sum_paired(double x0, double x1, double y0,
 // x1 = W - e, y0 = N - n, y1 = S - n, z0 = T - u, z1 = B - u) into vs[],
 // singular-point bits into sflags.  Without kGeneral the caller guarantees
 // that the point is on no face plane of the prism and between the two
-// planes of no axis.
-template <unsigned M, bool kGeneral>
+// planes of no axis; kGenX lifts the last condition for the x-planes (the
+// logarithms along x take their general form; the face angles need theirs
+// only between two pairs of planes, the singular points lie on face planes).
+template <unsigned M, bool kGeneral, bool kGenX = kGeneral>
 __device__ __forceinline__ void prism_fields(double x0, double x1, double y0, double y1,
                                              double z0, double z1, double (&vs)[FieldSet<M>::count],
                                              uint32_t &sflags, const dm::Tables &tab) {
@@ -331,7 +335,8 @@ __device__ __forceinline__ void prism_fields(double x0, double x1, double y0, do
 #pragma unroll
         for (int b = 0; b < 2; ++b) {
             if constexpr (F::Lx)
-                DLx[a][b] = log_diff<kGeneral>(x0, x1, r[0][a][b], r[1][a][b], ys[a] + zs[b], fx,
+                DLx[a][b] = log_diff<kGeneral || kGenX>(x0, x1, r[0][a][b], r[1][a][b],
+                                                        ys[a] + zs[b], fx,
                                                tab);
             if constexpr (F::Ly)
                 DLy[a][b] = log_diff<kGeneral>(y0, y1, r[a][0][b], r[a][1][b], xs[a] + zs[b], fy,
@@ -502,18 +507,39 @@ __device__ __forceinline__ bool needs_general(double e, double n, double u, doub
     return bx || by || bz || x0 * x1 == 0.0 || y0 * y1 == 0.0 || z0 * z1 == 0.0;
 }
 
+// Field sets with an instantiation for points between the x-planes only (on
+// no face plane): g_u alone and those evaluated by prism_fields.
+template <unsigned M>
+__host__ __device__ constexpr bool has_x_only() {
+    return M == 0x8u || (M != 0x2u && M != 0x4u && (M & 0xfu) != 0);
+}
+
+// Whether the point needs the general instantiation for a reason other than
+// lying between the x-planes (has_x_only field sets): between the y-planes
+// (g_u alone), or between the y- or z-planes or on a face plane.
+template <unsigned M>
+__device__ __forceinline__ bool needs_full_general(double e, double n, double u, double W,
+                                                   double E, double S, double N, double B,
+                                                   double T) {
+    const double x0 = E - e, x1 = W - e, y0 = N - n, y1 = S - n, z0 = T - u, z1 = B - u;
+    const bool by = y0 >= 0.0 && y1 < 0.0, bz = z0 >= 0.0 && z1 < 0.0;
+    if constexpr (M == 0x8u) return by;
+    return by || bz || x0 * x1 == 0.0 || y0 * y1 == 0.0 || z0 * z1 == 0.0;
+}
+
 // Add the fields of one prism (bounds W, E, S, N, B, T; grho = G * density)
 // at (e, n, u) to acc[]; OR singular-point bits into sflags.  Without
-// kGeneral, needs_general<M> must be false for the point.
-template <unsigned M, bool kGeneral>
+// kGeneral, needs_general<M> must be false for the point, or with kGenX
+// (has_x_only field sets) needs_full_general<M>.
+template <unsigned M, bool kGeneral, bool kGenX = kGeneral>
 __device__ __forceinline__ void prism_pair_path(double e, double n, double u, double W, double E,
                                                 double S, double N, double B, double T,
                                                 double grho, double (&acc)[FieldSet<M>::count],
                                                 uint32_t &sflags, const dm::Tables &tab) {
     using F = FieldSet<M>;
     if constexpr (M == 0x8u) {  // g_u alone
-        acc[0] += grho * gu_sum<false, kGeneral>(E - e, W - e, N - n, S - n, T - u, B - u, tab,
-                                                 nullptr);
+        acc[0] += grho * gu_sum<false, kGeneral, false, kGenX>(E - e, W - e, N - n, S - n, T - u,
+                                                               B - u, tab, nullptr);
     } else if constexpr (M == 0x2u) {  // g_e alone: K_e is K_u with (x, y, z) -> (y, z, x)
         acc[0] += grho * gu_sum<false, kGeneral>(N - n, S - n, T - u, B - u, E - e, W - e, tab,
                                                  nullptr);
@@ -528,7 +554,8 @@ __device__ __forceinline__ void prism_pair_path(double e, double n, double u, do
             if ((M >> f) & 1u) acc[F::slot(f)] += grho * K[f - 4];
     } else {
         double vs[F::count];
-        prism_fields<M, kGeneral>(E - e, W - e, N - n, S - n, T - u, B - u, vs, sflags, tab);
+        prism_fields<M, kGeneral, kGenX>(E - e, W - e, N - n, S - n, T - u, B - u, vs, sflags,
+                                         tab);
 #pragma unroll
         for (int s = 0; s < F::count; ++s) acc[s] += grho * vs[s];
     }
@@ -550,16 +577,20 @@ __device__ __forceinline__ void prism_pair(double e, double n, double u, double 
 // P has w, e, s, n, b, t, grho, int share and, with kMaySkip, int skip) at
 // (e, n, u).
 // Pass 1 evaluates the lean form for every prism that no lane of the wave
-// needs the general form for and marks the others in a wave-uniform
-// 128-bit mask; pass 2 evaluates the general form for the marked prisms.
-// Two loops keep the two forms' register allocations apart.  The order of
+// needs the general form for and marks the others in wave-uniform 128-bit
+// masks: those for which the x-planes alone made the lean form unusable
+// (has_x_only field sets; for the 64 points of a wave along a grid row,
+// the prisms of the 64 columns the points fall in) and the rest.  Pass 2
+// evaluates the first with the x-only form, pass 3 the rest with the
+// general form.
+// Separate loops keep the forms' register allocations apart.  The order of
 // the additions is fixed by the prisms and the points, so results are
 // deterministic.
 template <unsigned M, bool kMaySkip, class P>
 __device__ __forceinline__ void tile_pairs(const P *tile, int cnt, double e, double n, double u,
                                            double (&acc)[FieldSet<M>::count], uint32_t &sflags,
                                            const dm::Tables &tab) {
-    uint64_t general_lo = 0, general_hi = 0;
+    uint64_t general_lo = 0, general_hi = 0, xonly_lo = 0, xonly_hi = 0;
     // g_u alone: bottom vertex distances of the previous prism evaluated in
     // this pass, reused where P::share says an edge is shared.
     double rb[2][2];
@@ -573,10 +604,14 @@ __device__ __forceinline__ void tile_pairs(const P *tile, int cnt, double e, dou
             }
         }
         if (any_lane(needs_general<M>(e, n, u, sp.w, sp.e, sp.s, sp.n, sp.b, sp.t))) {
+            bool full = true;
+            if constexpr (has_x_only<M>())
+                full = any_lane(needs_full_general<M>(e, n, u, sp.w, sp.e, sp.s, sp.n, sp.b, sp.t));
+            uint64_t &lo = full ? general_lo : xonly_lo, &hi = full ? general_hi : xonly_hi;
             if (p < 64)
-                general_lo |= 1ull << p;
+                lo |= 1ull << p;
             else
-                general_hi |= 1ull << (p - 64);
+                hi |= 1ull << (p - 64);
             prev_lean = false;
             continue;
         }
@@ -589,6 +624,18 @@ __device__ __forceinline__ void tile_pairs(const P *tile, int cnt, double e, dou
         } else {
             prism_pair_path<M, false>(e, n, u, sp.w, sp.e, sp.s, sp.n, sp.b, sp.t, sp.grho, acc,
                                       sflags, tab);
+        }
+    }
+    if constexpr (has_x_only<M>()) {
+        for (int half = 0; half < 2; ++half) {
+            uint64_t m = half ? xonly_hi : xonly_lo;
+            while (m) {
+                const int p = 64 * half + __builtin_ctzll(m);
+                m &= m - 1;
+                const P sp = tile[p];
+                prism_pair_path<M, false, true>(e, n, u, sp.w, sp.e, sp.s, sp.n, sp.b, sp.t,
+                                                sp.grho, acc, sflags, tab);
+            }
         }
     }
     for (int half = 0; half < 2; ++half) {
