@@ -64,8 +64,9 @@ __device__ __forceinline__ void stage_mag(StagedMagPrism *tile, const double *__
 
 // Add the selected components of one prism at (e, n, u) to acc[].  Without
 // kGeneral, dev::needs_general<MagSet<C>::tensor> must be false for the
-// point (it is then on no face plane, so on no edge).
-template <unsigned C, bool kGeneral>
+// point (it is then on no face plane, so on no edge), or with kXOnly
+// dev::needs_full_general (the point may lie between the x-planes).
+template <unsigned C, bool kGeneral, bool kXOnly = false>
 __device__ __forceinline__ void magnetic_pair(double e, double n, double u,
                                               const StagedMagPrism &sp,
                                               double (&acc)[MagSet<C>::count],
@@ -83,6 +84,8 @@ __device__ __forceinline__ void magnetic_pair(double e, double n, double u,
 #pragma unroll
         for (int f = 4; f < 10; ++f)
             if ((M >> f) & 1u) K[f - 4] = vs[F::slot(f)];
+    } else if constexpr (kXOnly) {
+        dev::tensor_xonly<M>(x0, x1, y0, y1, z0, z1, K, tab);
     } else {
         dev::tensor_lean<M>(x0, x1, y0, y1, z0, z1, K, tab);
     }
@@ -105,24 +108,36 @@ __device__ __forceinline__ void magnetic_pair(double e, double n, double u,
     }
 }
 
-// The prisms tile[0, cnt) at (e, n, u), in two passes as dev::tile_pairs.
+// The prisms tile[0, cnt) at (e, n, u), in three passes as dev::tile_pairs
+// (lean, x-only, general).
 template <unsigned C>
 __device__ __forceinline__ void magnetic_tile(const StagedMagPrism *tile, int cnt, double e,
                                               double n, double u,
                                               double (&acc)[MagSet<C>::count],
                                               const dm::Tables &tab) {
     constexpr unsigned M = MagSet<C>::tensor;
-    uint64_t general_lo = 0, general_hi = 0;
+    uint64_t general_lo = 0, general_hi = 0, xonly_lo = 0, xonly_hi = 0;
     for (int p = 0; p < cnt; ++p) {
         const StagedMagPrism sp = tile[p];
         if (dev::any_lane(dev::needs_general<M>(e, n, u, sp.w, sp.e, sp.s, sp.n, sp.b, sp.t))) {
+            const bool full = dev::any_lane(
+                dev::needs_full_general<M>(e, n, u, sp.w, sp.e, sp.s, sp.n, sp.b, sp.t));
+            uint64_t &lo = full ? general_lo : xonly_lo, &hi = full ? general_hi : xonly_hi;
             if (p < 64)
-                general_lo |= 1ull << p;
+                lo |= 1ull << p;
             else
-                general_hi |= 1ull << (p - 64);
+                hi |= 1ull << (p - 64);
             continue;
         }
         magnetic_pair<C, false>(e, n, u, sp, acc, tab);
+    }
+    for (int half = 0; half < 2; ++half) {
+        uint64_t m = half ? xonly_hi : xonly_lo;
+        while (m) {
+            const int p = 64 * half + __builtin_ctzll(m);
+            m &= m - 1;
+            magnetic_pair<C, false, true>(e, n, u, tile[p], acc, tab);
+        }
     }
     for (int half = 0; half < 2; ++half) {
         uint64_t m = half ? general_hi : general_lo;

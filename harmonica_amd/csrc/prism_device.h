@@ -489,6 +489,90 @@ __device__ __forceinline__ void tensor_lean(double x0, double x1, double y0, dou
     (void)Z;
 }
 
+// tensor_lean for a point between the two x-planes of the prism but between
+// no other pair and on no face plane: the logarithms along x change branch
+// across the point, so K_nu is the sum of the four grouped x-logarithms of
+// prism_fields (general form), and the two x-face angles have opposite
+// signs, so K_ee is their difference taken angle by angle (neither face
+// is seen from inside: that needs the point between the y- and z-planes
+// too).  The other kernels keep the lean regrouping, which needs only their
+// own axis: K_en, K_eu one logarithm each, K_nn (and K_uu, or Laplace's
+// equation outside the prism) one arctangent each.
+template <unsigned TM>
+__device__ __forceinline__ void tensor_xonly(double x0, double x1, double y0, double y1,
+                                             double z0, double z1, double (&K)[6],
+                                             const dm::Tables &tab) {
+    using F = FieldSet<TM>;
+    const double xs[2] = {x0 * x0, x1 * x1}, ys[2] = {y0 * y0, y1 * y1};
+    const double zs[2] = {z0 * z0, z1 * z1};
+    const double Y[2] = {y0, y1}, Z[2] = {z0, z1};
+    double r[2][2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int k = 0; k < 2; ++k) r[i][j][k] = vertex_r(xs[i] + ys[j], z_tiny(zs[k]));
+    auto off = [&](auto coord, double a0) {
+        double num = 1.0, den = 1.0;
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+#pragma unroll
+                for (int k = 0; k < 2; ++k) {
+                    const double t = r[i][j][k] + fabs(coord(i, j, k));
+                    if ((i + j + k) & 1)
+                        den *= t;
+                    else
+                        num *= t;
+                }
+        return dm::flip_sign(dm::log_pos(dm::div(num, den), tab), dm::sign_mask(a0 < 0.0));
+    };
+    if constexpr (F::en) K[3] = off([&](int, int, int k) { return Z[k]; }, z0);
+    if constexpr (F::eu) K[4] = off([&](int, int j, int) { return Y[j]; }, y0);
+    if constexpr (F::nu) {
+        const unsigned fx = dm::sign_mask(x0 < 0.0);
+        double s = 0.0;
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                const double d =
+                    log_diff<true>(x0, x1, r[0][j][k], r[1][j][k], ys[j] + zs[k], fx, tab);
+                s = (j + k) & 1 ? s - d : s + d;
+            }
+        K[5] = s;
+    }
+    auto diag = [&](double N0, double D0, double N1, double D1) {
+        return dm::atan2_full<false>(fma(N1, D0, -N0 * D1), fma(D0, D1, N0 * N1), tab);
+    };
+    constexpr bool kLaplace = F::ee && F::nn && F::uu;
+    if constexpr (F::ee) {
+        const double o0 = face_sum<false>(y0, y1, z0, z1, x0, r[0][0][0], r[0][0][1], r[0][1][0],
+                                          r[0][1][1], tab);
+        const double o1 = face_sum<false>(y0, y1, z0, z1, x1, r[1][0][0], r[1][0][1], r[1][1][0],
+                                          r[1][1][1], tab);
+        K[0] = -(o0 - o1);
+    }
+    if constexpr (F::nn) {
+        double N0, D0, N1, D1;
+        face_nd(z0, z1, x0, x1, y0, r[0][0][0], r[1][0][0], r[0][0][1], r[1][0][1], N0, D0);
+        face_nd(z0, z1, x0, x1, y1, r[0][1][0], r[1][1][0], r[0][1][1], r[1][1][1], N1, D1);
+        K[1] = diag(N0, D0, N1, D1);
+    }
+    if constexpr (F::uu) {
+        if constexpr (kLaplace) {
+            K[2] = -(K[0] + K[1]);
+        } else {
+            double N0, D0, N1, D1;
+            face_nd(x0, x1, y0, y1, z0, r[0][0][0], r[0][1][0], r[1][0][0], r[1][1][0], N0, D0);
+            face_nd(x0, x1, y0, y1, z1, r[0][0][1], r[0][1][1], r[1][0][1], r[1][1][1], N1, D1);
+            K[2] = diag(N0, D0, N1, D1);
+        }
+    }
+}
+
 // Whether the point (e, n, u) needs the general instantiation for this
 // prism: for g_u, g_e, g_n alone when it lies between the two planes of one
 // of the two axes the logarithms run along; for other field sets when it
@@ -508,10 +592,11 @@ __device__ __forceinline__ bool needs_general(double e, double n, double u, doub
 }
 
 // Field sets with an instantiation for points between the x-planes only (on
-// no face plane): g_u alone and those evaluated by prism_fields.
+// no face plane): g_u alone, those evaluated by prism_fields and the
+// tensor-only sets (tensor_xonly).
 template <unsigned M>
 __host__ __device__ constexpr bool has_x_only() {
-    return M == 0x8u || (M != 0x2u && M != 0x4u && (M & 0xfu) != 0);
+    return M != 0x2u && M != 0x4u;
 }
 
 // Whether the point needs the general instantiation for a reason other than
@@ -548,7 +633,10 @@ __device__ __forceinline__ void prism_pair_path(double e, double n, double u, do
                                                  nullptr);
     } else if constexpr ((M & 0xfu) == 0 && !kGeneral) {  // tensor terms only
         double K[6];
-        tensor_lean<M>(E - e, W - e, N - n, S - n, T - u, B - u, K, tab);
+        if constexpr (kGenX)
+            tensor_xonly<M>(E - e, W - e, N - n, S - n, T - u, B - u, K, tab);
+        else
+            tensor_lean<M>(E - e, W - e, N - n, S - n, T - u, B - u, K, tab);
 #pragma unroll
         for (int f = 4; f < 10; ++f)
             if ((M >> f) & 1u) acc[F::slot(f)] += grho * K[f - 4];
