@@ -1577,11 +1577,14 @@ struct Fit {
                            (int64_t)batch, 0);
         HMX_LAUNCHED("fill_int_kernel");
         SideStream side;
-        const bool ahead = batch == 1 && n > 2 * (int64_t)kOuter;
-        if (ahead) {
-            HMX_HIP(hipStreamCreateWithFlags(&side.s2, hipStreamNonBlocking));
-            HMX_HIP(hipEventCreateWithFlags(&side.steps, hipEventDisableTiming));
-            HMX_HIP(hipEventCreateWithFlags(&side.rest, hipEventDisableTiming));
+        bool ahead = batch == 1 && n > 2 * (int64_t)kOuter;
+        // The lookahead is an optimisation: without its stream or events
+        // the factorisation runs on the fit's stream alone.
+        if (ahead && (hipStreamCreateWithFlags(&side.s2, hipStreamNonBlocking) != hipSuccess ||
+                      hipEventCreateWithFlags(&side.steps, hipEventDisableTiming) != hipSuccess ||
+                      hipEventCreateWithFlags(&side.rest, hipEventDisableTiming) != hipSuccess)) {
+            (void)hipGetLastError();  // clear the failed call's error
+            ahead = false;
         }
         bool rest_pending = false;
         auto join = [&]() -> int {  // the fit's stream waits for the last side update
