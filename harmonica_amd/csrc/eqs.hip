@@ -4,7 +4,7 @@
 // consecutive columns of the row-major (n_data, ld) matrix (coalesced
 // stores; the kernel is HBM-write bound).
 // predict: 256 observation points per workgroup, sources and coefficients
-// staged through LDS, FP64 accumulation (also for FP32 data).
+// read with scalar loads, FP64 accumulation (also for FP32 data).
 #include "devmath.h"
 #include "hmx_internal.h"
 
@@ -30,17 +30,16 @@ __global__ __launch_bounds__(256) void eqs_jacobian_kernel(
     }
 }
 
-template <class T>
-struct alignas(16) StagedSource {
-    T e, n, u, c;
-};
-
+// Sources are the same for every lane of a wavefront, so they are read
+// with scalar loads straight from global memory (the compiler turns the
+// wavefront-uniform reads of an unrolled loop into 64-byte scalar loads):
+// no LDS staging and no barriers.  Each lane adds its sources in index
+// order.
 template <class T>
 __global__ __launch_bounds__(kObsPerBlock) void eqs_predict_kernel(
     const T *__restrict__ e, const T *__restrict__ n, const T *__restrict__ u, int n_obs,
     const T *__restrict__ pe, const T *__restrict__ pn, const T *__restrict__ pu,
     const T *__restrict__ coefs, int64_t n_src, int64_t chunk_len, double *__restrict__ partial) {
-    __shared__ StagedSource<T> tile[kTile];
     const int i = blockIdx.x * kObsPerBlock + threadIdx.x;
     const bool valid = i < n_obs;
     const int ii = valid ? i : n_obs - 1;
@@ -48,23 +47,10 @@ __global__ __launch_bounds__(kObsPerBlock) void eqs_predict_kernel(
     double acc = 0.0;
     const int64_t c0 = (int64_t)blockIdx.y * chunk_len;
     const int64_t c1 = c0 + chunk_len < n_src ? c0 + chunk_len : n_src;
-    for (int64_t t0 = c0; t0 < c1; t0 += kTile) {
-        const int cnt = (int)(c1 - t0 < kTile ? c1 - t0 : kTile);
-        __syncthreads();
-        for (int q = threadIdx.x; q < cnt; q += blockDim.x) {
-            StagedSource<T> s;
-            s.e = pe[t0 + q];
-            s.n = pn[t0 + q];
-            s.u = pu[t0 + q];
-            s.c = coefs[t0 + q];
-            tile[q] = s;
-        }
-        __syncthreads();
-        for (int p = 0; p < cnt; ++p) {
-            const StagedSource<T> s = tile[p];
-            const T de = oe - s.e, dn = on - s.n, du = ou - s.u;
-            acc += (double)(s.c * inv_sqrt(de * de + dn * dn + du * du));
-        }
+#pragma unroll 8
+    for (int64_t j = c0; j < c1; ++j) {
+        const T de = oe - pe[j], dn = on - pn[j], du = ou - pu[j];
+        acc += (double)(coefs[j] * inv_sqrt(de * de + dn * dn + du * du));
     }
     if (valid) partial[(int64_t)blockIdx.y * n_obs + i] = acc;
 }
