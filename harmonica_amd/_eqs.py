@@ -208,6 +208,27 @@ def spatial_order(easting, northing, bits=16):
     return np.argsort(code, kind="stable")
 
 
+def strip_order(easting, northing, ratio=0.25, run=64):
+    """Permutation that walks the points in boustrophedon strips along
+    easting: strips of northing height h, easting ascending in even strips
+    and descending in odd ones.  h = sqrt(ratio * run * area / n) over the
+    points' bounding box, so that ``run`` consecutive points cover about
+    h x (h / ratio): ``ratio`` < 1 makes the runs narrower in northing than
+    in easting."""
+    e = np.asarray(easting, dtype=np.float64).ravel()
+    n = np.asarray(northing, dtype=np.float64).ravel()
+    if e.size < 2:
+        return np.arange(e.size)
+    span_e, span_n = np.ptp(e), np.ptp(n)
+    if span_n == 0.0:
+        return np.argsort(e, kind="stable")
+    if span_e == 0.0:
+        return np.argsort(n, kind="stable")
+    h = np.sqrt(ratio * run * span_e * span_n / e.size)
+    strip = np.floor((n - n.min()) / h).astype(np.int64)
+    return np.lexsort((np.where(strip % 2 == 0, e, -e), strip))
+
+
 #: Accuracy bar of the mixed-precision fits' predictions (relative to their
 #: largest value, against the FP64 fit) and the conjugate-gradient stop rule
 #: derived from it: the relative update of the data-space prediction in the

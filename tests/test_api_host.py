@@ -292,3 +292,47 @@ def test_magnetic_empty_coordinates():
                           MAG_PRISMS[0], [1.0, 1.0, 1.0], "b")
     assert len(b) == 3 and all(x.shape == (0, 3) for x in b)
     assert hm.prism_magnetic(([], [], []), MAG_PRISMS[0], [1.0, 1.0, 1.0], "b_n").shape == (0,)
+
+
+def test_strip_order_walks_grid_rows_boustrophedon():
+    """Strips one grid row high: even rows by ascending easting, odd rows
+    descending; the result is a permutation."""
+    from harmonica_amd._eqs import strip_order
+
+    e, n = np.meshgrid(np.arange(8.0), np.arange(4.0))
+    e, n = e.ravel(), n.ravel()
+    rng = np.random.default_rng(0)
+    shuffle = rng.permutation(e.size)
+    e, n = e[shuffle], n[shuffle]
+    ratio = 0.9**2 * e.size / (64 * 7 * 3)  # strip height 0.9
+    order = strip_order(e, n, ratio)
+    npt.assert_array_equal(np.sort(order), np.arange(e.size))
+    walked = np.column_stack([e[order], n[order]])
+    expected = [(x, y) for y in range(4) for x in (range(8) if y % 2 == 0 else range(7, -1, -1))]
+    npt.assert_array_equal(walked, np.array(expected, dtype=float))
+
+
+def test_strip_order_runs_are_compact():
+    """Runs of 64 consecutive random points stay within about h x h / ratio
+    (northing spread below two strip heights), unlike the input order."""
+    from harmonica_amd._eqs import strip_order
+
+    rng = np.random.default_rng(1)
+    e, n = rng.uniform(0, 2e4, 20000), rng.uniform(0, 2e4, 20000)
+    ratio = 0.25
+    h = np.sqrt(ratio * 64 * 2e4 * 2e4 / e.size)
+    order = strip_order(e, n, ratio)
+    npt.assert_array_equal(np.sort(order), np.arange(e.size))
+    runs = n[order][: e.size // 64 * 64].reshape(-1, 64)
+    spread = runs.max(axis=1) - runs.min(axis=1)
+    assert np.all(spread < 2 * h)
+    assert np.median(np.ptp(n[: e.size // 64 * 64].reshape(-1, 64), axis=1)) > 10 * h
+
+
+def test_strip_order_degenerate_inputs():
+    from harmonica_amd._eqs import strip_order
+
+    assert strip_order([], []).size == 0
+    npt.assert_array_equal(strip_order([5.0], [1.0]), [0])
+    npt.assert_array_equal(strip_order([3.0, 1.0, 2.0], [7.0, 7.0, 7.0]), [1, 2, 0])
+    npt.assert_array_equal(strip_order([7.0, 7.0, 7.0], [3.0, 1.0, 2.0]), [1, 2, 0])

@@ -1,7 +1,8 @@
 """
 Batched uneven prism models in one persistent launch: per-model results
 equal single-model prism_gravity (<= 1e-12 of max|g_z|) and the CPU oracle
-(1e-10); empty models and null prisms; HIP-graph replays identical to the
+(1e-10), with the points uploaded in strip order or in the caller's order;
+empty models and null prisms; HIP-graph replays identical to the
 eager run with eager work in between; autotuning keeps the results.
 """
 
@@ -37,11 +38,12 @@ def _models(seed=3, n_models=14):
     return models
 
 
-def test_batched_matches_single_models(hm):
+@pytest.mark.parametrize("strip_ratio", [0.25, None], ids=["strip_order", "caller_order"])
+def test_batched_matches_single_models(hm, strip_ratio):
     from harmonica_amd._batched import BatchedPrismGz
 
     models = _models()
-    batch = BatchedPrismGz(models, prisms_per_item=512, blocks_per_cu=4)
+    batch = BatchedPrismGz(models, prisms_per_item=512, blocks_per_cu=4, strip_ratio=strip_ratio)
     batch.run()
     results = batch.split()
     for (c, p, d), got in zip(models, results):
