@@ -75,6 +75,9 @@ int hmx_forward_workspace_bytes(int64_t n_obs, int64_t n_sources, uint32_t field
     if (cu < 0) return cu;
     int64_t chunks = hmx::forward_chunks(n_obs, n_sources, cu);
     *bytes = (size_t)chunks * (size_t)nf * (size_t)n_obs * sizeof(double);
+    // the prism paths' point order, after the partial sums
+    if (hmx::use_point_order(n_obs, n_sources))
+        *bytes = hmx::point_order_offset(*bytes) + hmx::point_order_bytes(n_obs);
     return HMX_OK;
 }
 

@@ -69,6 +69,33 @@ inline int64_t forward_chunks(int64_t n_obs, int64_t n_src, int n_cu) {
 
 inline int popcount10(uint32_t m) { return __builtin_popcount(m & HMX_ALL_FIELDS); }
 
+// Wavefront-coherent point order of the prism kernels (point_order.hip):
+// gathered copies of the coordinates in boustrophedon strip order and the
+// permutation perm (sorted position -> caller's index) for the reduction.
+struct PointOrder {
+    const double *e, *n, *u;
+    const int *perm;
+};
+
+// Whether a prism call sorts its points: enough points for the strips to
+// matter and enough pairs that the sort's dozen small launches are
+// negligible next to the pair loop.
+inline bool use_point_order(int64_t n_obs, int64_t n_src) {
+    return n_obs >= 2048 && n_obs * n_src >= ((int64_t)1 << 30);
+}
+
+// Bytes of the order's workspace region for n_obs points.
+size_t point_order_bytes(int64_t n_obs);
+
+// Sort the points into the region `ws` (point_order_bytes(n_obs) bytes,
+// 256-byte aligned) on stream s.
+int point_order(const double *e, const double *n, const double *u, int64_t n_obs, void *ws,
+                PointOrder *po, hipStream_t s);
+
+// Offset of the order region in a forward workspace whose partial sums take
+// partial_bytes.
+inline size_t point_order_offset(size_t partial_bytes) { return (partial_bytes + 255) & ~(size_t)255; }
+
 // Least squares min |G c - y|^2 (+ alpha |c|^2, per filter) through the
 // one-sided Jacobi SVD (lsq_svd.hip) of A (L x k, column-major, L >= k,
 // overwritten): A = G (n x m, transposed = 0) or A = G^T (transposed = 1).

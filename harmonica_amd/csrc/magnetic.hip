@@ -181,16 +181,18 @@ __global__ __launch_bounds__(kObsPerBlock) void prism_magnetic_kernel(
     }
 }
 
-// out[s][i] = 1e9 * (sum over chunks in order of partial[c][s][i])  (T -> nT).
+// out[s][perm[i]] = 1e9 * (sum over chunks in order of partial[c][s][i])
+// (T -> nT; perm NULL: the identity).
 __global__ void reduce_magnetic_kernel(const double *__restrict__ partial, int64_t chunks, int nc,
-                                       int64_t n_obs, double *__restrict__ out) {
+                                       int64_t n_obs, const int *__restrict__ perm,
+                                       double *__restrict__ out) {
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (int64_t)nc * n_obs) return;
     const int s = (int)(idx / n_obs);
     const int64_t i = idx - (int64_t)s * n_obs;
     double sum = 0.0;
     for (int64_t c = 0; c < chunks; ++c) sum += partial[(c * nc + s) * n_obs + i];
-    out[idx] = 1e9 * sum;
+    out[(int64_t)s * n_obs + (perm ? perm[i] : i)] = 1e9 * sum;
 }
 
 __global__ void fill_zero_mag_kernel(double *__restrict__ out, int64_t n) {
@@ -229,6 +231,9 @@ int hmx_prism_magnetic_workspace_bytes(int64_t n_obs, int64_t n_prisms, uint32_t
     const int64_t chunks = hmx::forward_chunks(n_obs, n_prisms, cu);
     *bytes = (size_t)chunks * (size_t)__builtin_popcount(component_mask) * (size_t)n_obs *
              sizeof(double);
+    // the point order, after the partial sums
+    if (hmx::use_point_order(n_obs, n_prisms))
+        *bytes = hmx::point_order_offset(*bytes) + hmx::point_order_bytes(n_obs);
     return HMX_OK;
 }
 
@@ -260,11 +265,26 @@ int hmx_prism_magnetic(const double *easting, const double *northing, const doub
     const int cu = hmx::device_cu_count();
     if (cu < 0) return cu;
     const int64_t chunks = hmx::forward_chunks(n_obs, n_prisms, cu);
-    const size_t need = (size_t)chunks * (size_t)nc * (size_t)n_obs * sizeof(double);
+    const size_t partial_bytes = (size_t)chunks * (size_t)nc * (size_t)n_obs * sizeof(double);
+    const bool order = hmx::use_point_order(n_obs, n_prisms);
+    const size_t order_at = hmx::point_order_offset(partial_bytes);
+    const size_t need = order ? order_at + hmx::point_order_bytes(n_obs) : partial_bytes;
     if (!workspace || workspace_bytes < need)
         return hmx::fail(HMX_ERR_ARG, "workspace too small: %zu < %zu bytes", workspace_bytes,
                          need);
     double *partial = static_cast<double *>(workspace);
+    // points in wavefront-coherent order (see point_order.hip)
+    const int *perm = nullptr;
+    if (order) {
+        hmx::PointOrder po;
+        const int ost = hmx::point_order(easting, northing, upward, n_obs,
+                                         static_cast<char *>(workspace) + order_at, &po, st);
+        if (ost) return ost;
+        easting = po.e;
+        northing = po.n;
+        upward = po.u;
+        perm = po.perm;
+    }
     int rc;
     switch (component_mask) {
 #define HMX_MAG_CASE(C)                                                                       \
@@ -287,7 +307,7 @@ int hmx_prism_magnetic(const double *easting, const double *northing, const doub
     if (rc) return rc;
     const int64_t total = (int64_t)nc * n_obs;
     hipLaunchKernelGGL(hmx::reduce_magnetic_kernel, dim3((unsigned)((total + 255) / 256)),
-                       dim3(256), 0, st, partial, chunks, nc, n_obs, out);
+                       dim3(256), 0, st, partial, chunks, nc, n_obs, perm, out);
     HMX_LAUNCHED("reduce_magnetic_kernel");
     return HMX_OK;
 }

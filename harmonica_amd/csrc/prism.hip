@@ -183,9 +183,11 @@ __global__ __launch_bounds__(kObsPerBlock) void singular_scan_kernel(
     if (valid && found) atomicOr(flags, found);
 }
 
-// out[s][i] = units(sum over chunks in order of partial[c][s][i]).
+// out[s][perm[i]] = units(sum over chunks in order of partial[c][s][i])
+// (perm NULL: the identity).
 __global__ void reduce_chunks_kernel(const double *__restrict__ partial, int64_t chunks, int nf,
-                                     int64_t n_obs, uint32_t mask, double *__restrict__ out) {
+                                     int64_t n_obs, uint32_t mask, const int *__restrict__ perm,
+                                     double *__restrict__ out) {
     int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (int64_t)nf * n_obs) return;
     int s = (int)(idx / n_obs);
@@ -196,7 +198,7 @@ __global__ void reduce_chunks_kernel(const double *__restrict__ partial, int64_t
     for (; f < 10; ++f)
         if ((mask >> f) & 1u)
             if (++seen == s) break;
-    out[idx] = dev::to_harmonica_units(f, sum);
+    out[(int64_t)s * n_obs + (perm ? perm[i] : i)] = dev::to_harmonica_units(f, sum);
 }
 
 // Zero fill with a kernel (stays valid inside captured HIP graphs).
@@ -218,10 +220,10 @@ int launch_forward(const double *oe, const double *on, const double *ou, int64_t
 }
 
 int launch_reduce(const double *partial, int64_t chunks, int nf, int64_t n_obs, uint32_t mask,
-                  double *out, hipStream_t stream) {
+                  const int *perm, double *out, hipStream_t stream) {
     int64_t total = (int64_t)nf * n_obs;
     hipLaunchKernelGGL(reduce_chunks_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
-                       stream, partial, chunks, nf, n_obs, mask, out);
+                       stream, partial, chunks, nf, n_obs, mask, perm, out);
     HMX_LAUNCHED("reduce_chunks_kernel");
     return HMX_OK;
 }
@@ -262,12 +264,15 @@ bool has_kernel(uint32_t mask) {
 // Run the forward model for any field mask: one launch when the mask has
 // its own kernel, else one launch per field reusing the workspace.
 // chunk_cap > 0 limits the number of source chunks (so that the partial
-// sums fit a fixed scratch buffer).
+// sums fit a fixed scratch buffer).  With may_order, calls that pass
+// use_point_order evaluate the points in wavefront-coherent order, sorted
+// into the workspace after the room for all fields' partial sums (the
+// layout hmx_forward_workspace_bytes sizes).
 template <class Src>
 int run_forward(const double *oe, const double *on, const double *ou, int64_t n_obs,
                 const Src &src, int64_t n_src, uint32_t mask, double *out, uint32_t *flags,
                 void *workspace, size_t workspace_bytes, hipStream_t stream,
-                int64_t chunk_cap = 0) {
+                int64_t chunk_cap = 0, bool may_order = true) {
     if (n_obs <= 0) return HMX_OK;
     int cu = device_cu_count();
     if (cu < 0) return cu;
@@ -284,17 +289,29 @@ int run_forward(const double *oe, const double *on, const double *ou, int64_t n_
     }
     const size_t need_one = (size_t)chunks * (size_t)n_obs * sizeof(double);
     double *partial = static_cast<double *>(workspace);
+    const bool order = may_order && use_point_order(n_obs, n_src);
+    const size_t order_at = point_order_offset(need_one * nf);
+    const size_t need = order ? order_at + point_order_bytes(n_obs)
+                              : (has_kernel(mask) ? need_one * nf : need_one);
+    if (!workspace || workspace_bytes < need)
+        return fail(HMX_ERR_ARG, "workspace too small: %zu < %zu bytes", workspace_bytes, need);
+    const int *perm = nullptr;
+    if (order) {
+        PointOrder po;
+        int st = point_order(oe, on, ou, n_obs, static_cast<char *>(workspace) + order_at, &po,
+                             stream);
+        if (st) return st;
+        oe = po.e;
+        on = po.n;
+        ou = po.u;
+        perm = po.perm;
+    }
     if (has_kernel(mask)) {
-        if (!workspace || workspace_bytes < need_one * nf)
-            return fail(HMX_ERR_ARG, "workspace too small: %zu < %zu bytes", workspace_bytes,
-                        need_one * nf);
         int st = dispatch_mask<Src>(mask, oe, on, ou, n_obs, src, n_src, chunks, partial, flags,
                                     stream);
         if (st) return st;
-        return launch_reduce(partial, chunks, nf, n_obs, mask, out, stream);
+        return launch_reduce(partial, chunks, nf, n_obs, mask, perm, out, stream);
     }
-    if (!workspace || workspace_bytes < need_one)
-        return fail(HMX_ERR_ARG, "workspace too small: %zu < %zu bytes", workspace_bytes, need_one);
     int s = 0;
     for (int f = 0; f < 10; ++f) {
         if (!((mask >> f) & 1u)) continue;
@@ -302,7 +319,7 @@ int run_forward(const double *oe, const double *on, const double *ou, int64_t n_
         int st = dispatch_mask<Src>(m, oe, on, ou, n_obs, src, n_src, chunks, partial, flags,
                                     stream);
         if (st) return st;
-        st = launch_reduce(partial, chunks, 1, n_obs, m, out + (int64_t)s * n_obs, stream);
+        st = launch_reduce(partial, chunks, 1, n_obs, m, perm, out + (int64_t)s * n_obs, stream);
         if (st) return st;
         ++s;
     }
@@ -379,7 +396,7 @@ int hmx_prism_gravity_static(const double *easting, const double *northing,
     ExplicitSource src{prisms, density};
     return run_forward(easting, northing, upward, n_obs, src, n_prisms, field_mask, out,
                        singular_flags, ws, (size_t)kStaticScratchDoubles * sizeof(double),
-                       (hipStream_t)stream, kStaticScratchDoubles / per_chunk);
+                       (hipStream_t)stream, kStaticScratchDoubles / per_chunk, false);
 }
 
 int hmx_prism_gravity(const double *easting, const double *northing, const double *upward,

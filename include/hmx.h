@@ -115,7 +115,11 @@ int hmx_forward_workspace_bytes(int64_t n_obs, int64_t n_sources, uint32_t field
  *             observation point is a singular point of tensor field f of some
  *             prism (gravity.py:239-269); that value is NaN.
  * Null prisms are NOT discarded here (the Python layer does it as
- * _discard_null_prisms, gravity.py:452-486, does). */
+ * _discard_null_prisms, gravity.py:452-486, does).
+ * Calls with at least 2048 points and 2^30 (point, prism) pairs evaluate the
+ * points in a wavefront-coherent strip order sorted into the workspace
+ * (also hmx_prism_layer_gravity and hmx_prism_magnetic); out is in the
+ * caller's order either way. */
 int hmx_prism_gravity(const double *easting, const double *northing, const double *upward,
                       int64_t n_obs, const double *prisms, const double *density,
                       int64_t n_prisms, uint32_t field_mask, double *out,
