@@ -731,6 +731,8 @@ __global__ __launch_bounds__(1024) void trsv_fwd_sync_kernel(
     const int64_t c0 = (int64_t)b * kNB, c = c0 + t;
     const bool cv = c < m;
     block_to_lds(Dinv + (int64_t)b * kNB * kNB, dinv);
+    // the right-hand side entry, loaded now rather than on the chain
+    const double rc = (g == 0 && cv) ? r[c] : 0.0;
     double u[16];
     // Columns past m (last block only) read column m - 1: finite values
     // whose sums are dropped (z = 0 there).
@@ -755,7 +757,7 @@ __global__ __launch_bounds__(1024) void trsv_fwd_sync_kernel(
         double s = 0.0;
 #pragma unroll
         for (int q = 0; q < 8; ++q) s += part[q][t];
-        z[t] = cv ? r[c] - s : 0.0;
+        z[t] = cv ? rc - s : 0.0;
     }
     __syncthreads();
     double d = 0.0;  // (Dinv_b^T z)[t] = sum_{l <= t} Dinv_b[l][t] z[l]
@@ -790,6 +792,11 @@ __global__ __launch_bounds__(1024) void trsv_bwd_sync_kernel(
     const int t = threadIdx.x & (kNB - 1), g = threadIdx.x >> 7;
     const int64_t r0 = (int64_t)b * kNB;
     block_to_lds(DinvT + (int64_t)b * kNB * kNB, dinvt);
+    // Lanes 8r of wavefront v finish row 8 v + r of the block (see the sum
+    // over lanes below); its right-hand side entry, loaded now rather than
+    // on the chain.
+    const int64_t my_row = r0 + 8 * v + (lane >> 3);
+    const double yr = ((lane & 7) == 0 && my_row < m) ? y[my_row] : 0.0;
     double u[16];
     // Rows and columns past m read row or column m - 1: finite values that
     // meet x = 0 (columns) or whose sums are dropped (rows, z = 0 there).
@@ -819,17 +826,30 @@ __global__ __launch_bounds__(1024) void trsv_bwd_sync_kernel(
         for (int i = 0; i < 8; ++i) acc[i] = fma(u[2 * i + 1], x1, fma(u[2 * i], x0, acc[i]));
         if (k - 1 > b) load_u(k - 1);
     }
+    // Sums over the 64 lanes of the eight rows' partial sums, reduce-scatter
+    // style: three exchange-and-add steps (lane offsets 32, 16, 8) halve the
+    // rows a lane holds, keeping row 4 b5 + 2 b4 + b3 (b the lane's bits),
+    // then three steps add within groups of 8 lanes, so that lanes 8r..8r+7
+    // hold row r's sum: 10 shuffles on the chain instead of 48.
+    double a4[4], a2[2];
+    {
+        const bool hi = lane & 32;
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1)
-#pragma unroll
-        for (int i = 0; i < 8; ++i) acc[i] += __shfl_xor(acc[i], off, 64);
-    if (lane < 8) {
-        double s = acc[0];
-#pragma unroll
-        for (int i = 1; i < 8; ++i) s = lane == i ? acc[i] : s;
-        const int64_t row = r0 + 8 * v + lane;
-        z[8 * v + lane] = row < m ? y[row] - s : 0.0;
+        for (int i = 0; i < 4; ++i)
+            a4[i] = (hi ? acc[4 + i] : acc[i]) + __shfl_xor(hi ? acc[i] : acc[4 + i], 32, 64);
     }
+    {
+        const bool hi = lane & 16;
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+            a2[i] = (hi ? a4[2 + i] : a4[i]) + __shfl_xor(hi ? a4[i] : a4[2 + i], 16, 64);
+    }
+    const bool hi8 = lane & 8;
+    double s = (hi8 ? a2[1] : a2[0]) + __shfl_xor(hi8 ? a2[0] : a2[1], 8, 64);
+    s += __shfl_xor(s, 4, 64);
+    s += __shfl_xor(s, 2, 64);
+    s += __shfl_xor(s, 1, 64);
+    if ((lane & 7) == 0) z[8 * v + (lane >> 3)] = my_row < m ? yr - s : 0.0;
     __builtin_amdgcn_s_waitcnt(0);  // this wave's part of the DinvT_b copy
     __syncthreads();
     double d = 0.0;  // (Dinv_b z)[t] = sum_{l >= t} DinvT_b[l][t] z[l]
