@@ -136,3 +136,20 @@ def test_sorted_magnetic(hm, model, kind):
         expected = np.asarray(oracle.prism_magnetic(sample, prisms, mag, "b"))
         for k in range(3):
             assert_close(got[k][idx], expected[k], 1e-10, err_msg=f"oracle component {k}")
+
+
+@pytest.mark.parametrize("n_obs", [2048, 5003])
+def test_sorted_ragged_point_counts(hm, model, n_obs):
+    """The smallest sorted call (2048 points) and a point count that leaves
+    the last radix tile and the last workgroup partial."""
+    prisms, density, _ = model
+    rng = np.random.default_rng(n_obs)
+    coords = (rng.uniform(0, 2e4, n_obs), rng.uniform(0, 2e4, n_obs), rng.uniform(-800, 400, n_obs))
+    m = -(-(1 << 30) // n_obs)  # enough prisms for 2^30 pairs (repeated if needed)
+    p, d = np.resize(prisms, (m, 6)), np.resize(density, m)
+    got = _fields(hm, coords, p, d, TEN)
+    parts = [_fields(hm, tuple(c[k:k + PIECE] for c in coords), p, d, TEN)
+             for k in range(0, n_obs, PIECE)]
+    ref = np.concatenate(parts, axis=-1)
+    for k, f in enumerate(TEN):
+        assert_close(got[k], ref[k], 1e-13, err_msg=f"{n_obs} points, {f}")
