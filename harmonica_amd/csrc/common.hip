@@ -68,6 +68,8 @@ int hmx_forward_workspace_bytes(int64_t n_obs, int64_t n_sources, uint32_t field
                                 size_t *bytes) {
     if (!bytes) return hmx::fail(HMX_ERR_ARG, "bytes is NULL");
     if (n_obs < 0 || n_sources < 0) return hmx::fail(HMX_ERR_ARG, "negative size");
+    if (n_obs > INT32_MAX || n_sources > INT32_MAX)
+        return hmx::fail(HMX_ERR_ARG, "more than 2^31-1 observation points or sources");
     int nf = hmx::popcount10(field_mask);
     if (nf == 0 || (field_mask & ~HMX_ALL_FIELDS))
         return hmx::fail(HMX_ERR_ARG, "invalid field mask 0x%x", field_mask);

@@ -81,7 +81,8 @@ struct PointOrder {
 // matter and enough pairs that the sort's dozen small launches are
 // negligible next to the pair loop.
 inline bool use_point_order(int64_t n_obs, int64_t n_src) {
-    return n_obs >= 2048 && n_obs * n_src >= ((int64_t)1 << 30);
+    // n_obs * n_src >= 2^30, without forming the product
+    return n_obs >= 2048 && n_src >= ((((int64_t)1 << 30) + n_obs - 1) / n_obs);
 }
 
 // Bytes of the order's workspace region for n_obs points.

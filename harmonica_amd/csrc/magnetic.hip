@@ -224,6 +224,8 @@ int hmx_prism_magnetic_workspace_bytes(int64_t n_obs, int64_t n_prisms, uint32_t
     hmx::clear_error();
     if (!bytes) return hmx::fail(HMX_ERR_ARG, "bytes is NULL");
     if (n_obs < 0 || n_prisms < 0) return hmx::fail(HMX_ERR_ARG, "negative size");
+    if (n_obs > INT32_MAX || n_prisms > INT32_MAX)
+        return hmx::fail(HMX_ERR_ARG, "more than 2^31-1 observation points or prisms");
     if (!hmx::valid_components(component_mask))
         return hmx::fail(HMX_ERR_ARG, "invalid component mask 0x%x", component_mask);
     const int cu = hmx::device_cu_count();

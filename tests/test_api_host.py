@@ -336,3 +336,18 @@ def test_strip_order_degenerate_inputs():
     npt.assert_array_equal(strip_order([5.0], [1.0]), [0])
     npt.assert_array_equal(strip_order([3.0, 1.0, 2.0], [7.0, 7.0, 7.0]), [1, 2, 0])
     npt.assert_array_equal(strip_order([7.0, 7.0, 7.0], [3.0, 1.0, 2.0]), [1, 2, 0])
+
+
+def test_forward_workspace_queries_reject_sizes_past_int32():
+    """The workspace queries refuse what the entry points refuse (2^31 or
+    more points or sources) before touching a device, so no size product
+    can overflow."""
+    L = _lib.lib()
+    nbytes = ctypes.c_size_t(0)
+    for n_obs, n_src in ((1 << 31, 10), (10, 1 << 31), (1 << 40, 1 << 40)):
+        st = L.hmx_forward_workspace_bytes(n_obs, n_src, 0x8, ctypes.byref(nbytes))
+        assert st == _lib.HMX_ERR_ARG
+        assert b"2^31-1" in L.hmx_last_error()
+        st = L.hmx_prism_magnetic_workspace_bytes(n_obs, n_src, 0x7, ctypes.byref(nbytes))
+        assert st == _lib.HMX_ERR_ARG
+        assert b"2^31-1" in L.hmx_last_error()
