@@ -1100,6 +1100,9 @@ __global__ void add_diag_kernel(double *__restrict__ A, int64_t m, double tau) {
 // which takes twice the rows of the bf16 MFMA in the same issue time.
 constexpr int kSlices = 4;
 constexpr int kFoldRows = 1024;  // rows per group (one scale per group and column)
+// Slice units (k-groups) per chunk of the gram kernel at most: the slices
+// are padded with zero rows to a whole number of them.
+constexpr int kSplitChunkUnits = 8;
 
 struct SplitBf16 {
     using unit_t = bf16x8_t;  // one slice of 8 rows of one column
@@ -1268,13 +1271,20 @@ __global__ __launch_bounds__(512) void gram_split_kernel(const typename Tr::unit
                                                         int64_t tiles_n) {
     constexpr int kNO = O1 - O0 + 1;  // orders accumulated here
     constexpr int kNS = O1 + 1;       // slices needed on each side
-    constexpr int kFoldChunks = fold_chunks<Tr>();
+    // k-steps of 4 k-groups (one MFMA deep) per chunk: two when two slices
+    // per side leave room in LDS, so that each wait for a chunk's loads is
+    // covered by twice the MFMAs.
+    constexpr int kSub = kNS <= 2 ? 2 : 1;
+    constexpr int kKG = 4 * kSub;  // k-groups per chunk
+    static_assert(kSplitChunkUnits % kKG == 0, "slices are padded to whole chunks");
+    constexpr int kFoldChunks = fold_chunks<Tr>() / kSub;
+    static_assert(kFoldChunks % 2 == 0, "every fold starts in the first buffer");
     using unit_t = typename Tr::unit_t;
     using acc_t = typename Tr::acc_t;
     // Two staging buffers as separate arrays, so that the compiler can tell
     // the buffer being filled from the one being read.
-    __shared__ unit_t lds0[2][kNS][4][kST];
-    __shared__ unit_t lds1[2][kNS][4][kST];
+    __shared__ unit_t lds0[2][kNS][kKG][kST];
+    __shared__ unit_t lds1[2][kNS][kKG][kST];
     int64_t rb, cb;
     if (!upper_tile_xcd(blockIdx.x, tiles_n, rb, cb)) return;
     const int64_t r0 = rb * kST, c0 = cb * kST;
@@ -1294,44 +1304,53 @@ __global__ __launch_bounds__(512) void gram_split_kernel(const typename Tr::unit
     // the products they enter belong to output columns past m, never stored.
     const int kg = threadIdx.x >> 7, col = threadIdx.x & (kST - 1);
     const int col0 = col & ~63;
-    auto issue = [&](int ch, unit_t (&L)[2][kNS][4][kST]) {
-        const int64_t g = 4 * (int64_t)ch + kg;
+    // n_groups is a multiple of kSplitChunkUnits, so every chunk is whole.
+    auto issue = [&](int ch, unit_t (&L)[2][kNS][kKG][kST]) {
 #pragma unroll
-        for (int side = 0; side < 2; ++side) {
-            int64_t gc = (side ? c0 : r0) + col;
-            gc = gc < m ? gc : m - 1;
+        for (int q = 0; q < kSub; ++q) {
+            const int64_t g = (int64_t)kKG * ch + 4 * q + kg;
 #pragma unroll
-            for (int t = 0; t < kNS; ++t)
-                __builtin_amdgcn_global_load_lds(
-                    (const void *)(S + ((int64_t)t * n_groups + g) * m + gc),
-                    (__attribute__((address_space(3))) void *)&L[side][t][kg][col0], 16, 0, 0);
+            for (int side = 0; side < 2; ++side) {
+                int64_t gc = (side ? c0 : r0) + col;
+                gc = gc < m ? gc : m - 1;
+#pragma unroll
+                for (int t = 0; t < kNS; ++t)
+                    __builtin_amdgcn_global_load_lds(
+                        (const void *)(S + ((int64_t)t * n_groups + g) * m + gc),
+                        (__attribute__((address_space(3))) void *)&L[side][t][4 * q + kg][col0],
+                        16, 0, 0);
+            }
         }
     };
     // MFMAs of one chunk from the staged buffer L.
-    auto compute = [&](const unit_t (&L)[2][kNS][4][kST]) {
+    auto compute = [&](const unit_t (&L)[2][kNS][kKG][kST]) {
 #pragma unroll
-        for (int o = 0; o < kNO; ++o)
+        for (int q = 0; q < kSub; ++q) {
+            const int kq = 4 * q + (lane >> 4);
 #pragma unroll
-            for (int sa = 0; sa <= O0 + o; ++sa) {
-                const int sb = O0 + o - sa;
-                unit_t a[4], bb[2];
+            for (int o = 0; o < kNO; ++o)
 #pragma unroll
-                for (int t = 0; t < 4; ++t) a[t] = L[0][sa][lane >> 4][64 * wr + 16 * t + (lane & 15)];
+                for (int sa = 0; sa <= O0 + o; ++sa) {
+                    const int sb = O0 + o - sa;
+                    unit_t a[4], bb[2];
 #pragma unroll
-                for (int t = 0; t < 2; ++t) bb[t] = L[1][sb][lane >> 4][32 * wc + 16 * t + (lane & 15)];
+                    for (int t = 0; t < 4; ++t) a[t] = L[0][sa][kq][64 * wr + 16 * t + (lane & 15)];
 #pragma unroll
-                for (int ti = 0; ti < 4; ++ti)
+                    for (int t = 0; t < 2; ++t) bb[t] = L[1][sb][kq][32 * wc + 16 * t + (lane & 15)];
 #pragma unroll
-                    for (int tj = 0; tj < 2; ++tj)
-                        acc[o][ti][tj] = Tr::mfma(a[ti], bb[tj], acc[o][ti][tj]);
-            }
+                    for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+                        for (int tj = 0; tj < 2; ++tj)
+                            acc[o][ti][tj] = Tr::mfma(a[ti], bb[tj], acc[o][ti][tj]);
+                }
+        }
     };
     // The chunk's loads complete (this wave's), then every wave's.
     auto arrive = [&]() {
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
     };
-    const int chunks = __builtin_amdgcn_readfirstlane((int)(n_groups / 4));
+    const int chunks = __builtin_amdgcn_readfirstlane((int)(n_groups / kKG));
     if (chunks > 0) issue(0, lds0);
     arrive();
     for (int g0 = 0; g0 < chunks; g0 += kFoldChunks) {
@@ -1439,10 +1458,11 @@ inline size_t fit_x_bytes(int64_t n_data, int64_t n_src, int mode) {
     size_t x_bytes = n * m * sizeof(double);  // X = W^1/2 J S
     if (mode == HMX_FIT_FP32_MFMA) x_bytes = n * m * sizeof(float);
     if (mode == HMX_FIT_BF16_SPLIT || mode == HMX_FIT_INT8_SPLIT) {
-        // slices (16-byte units of 8 or 16 rows, padded to whole chunks of 4
-        // units) and the group scales
+        // slices (16-byte units of 8 or 16 rows, padded to whole chunks of
+        // kSplitChunkUnits units) and the group scales
         const size_t rows = mode == HMX_FIT_INT8_SPLIT ? 16 : 8;
-        const size_t n_units = (n + 4 * rows - 1) / (4 * rows) * 4;
+        const size_t n_units = (n + kSplitChunkUnits * rows - 1) / (kSplitChunkUnits * rows) *
+                               kSplitChunkUnits;
         x_bytes = kSlices * n_units * m * 16 +
                   (n_units * rows + kFoldRows - 1) / kFoldRows * m * sizeof(double);
     }
@@ -2121,9 +2141,10 @@ int hmx_eqs_fit(const double *easting, const double *northing, const double *upw
     const bool split = precision_mode == HMX_FIT_BF16_SPLIT || int8;
     const size_t nblk = (size_t)((m + kNB - 1) / kNB);
     // split modes: 16-byte slice units of 8 (bf16) or 16 (int8) rows, padded
-    // to whole chunks of 4 units
+    // to whole chunks of kSplitChunkUnits units
     const int64_t unit_rows = int8 ? 16 : 8;
-    const int64_t n_groups = (nd + 4 * unit_rows - 1) / (4 * unit_rows) * 4;
+    const int64_t n_groups = (nd + kSplitChunkUnits * unit_rows - 1) /
+                             (kSplitChunkUnits * unit_rows) * kSplitChunkUnits;
 
     char *ws = static_cast<char *>(workspace);
     Fit f;

@@ -146,9 +146,9 @@ def test_fit_workspace_bytes_per_mode():
     assert st == 0 and b32 == common + m * m * 8 + region(n, m, n * m * 4)
     n_fold = 5
     st, bb = nbytes(n, m, _lib.HMX_FIT_BF16_SPLIT)
-    assert st == 0 and bb == common + m * m * 8 + region(n, m, 4 * 5024 * m * 2 + n_fold * m * 8)
+    assert st == 0 and bb == common + m * m * 8 + region(n, m, 4 * 5056 * m * 2 + n_fold * m * 8)
     st, bi = nbytes(n, m, _lib.HMX_FIT_INT8_SPLIT)
-    assert st == 0 and bi == common + m * m * 8 + region(n, m, 4 * 5056 * m + n_fold * m * 8)
+    assert st == 0 and bi == common + m * m * 8 + region(n, m, 4 * 5120 * m + n_fold * m * 8)
     for mode, b in ((_lib.HMX_FIT_FP64, b64), (_lib.HMX_FIT_INT8_SPLIT, bi)):
         assert nbytes(n, m, mode | _lib.HMX_FIT_NO_DAMPING) == (0, b)
     # beyond the SVD route's sizes the mode's own jacobian only
@@ -156,6 +156,6 @@ def test_fit_workspace_bytes_per_mode():
     st, b64 = nbytes(n, m, _lib.HMX_FIT_FP64)
     common = b64 - n * m * 8
     st, bi = nbytes(n, m, _lib.HMX_FIT_INT8_SPLIT)
-    assert st == 0 and bi == common + m * m * 8 + 4 * (20032 // 16) * 16 * m + 20 * m * 8
+    assert st == 0 and bi == common + m * m * 8 + 4 * (20096 // 16) * 16 * m + 20 * m * 8
     assert nbytes(n, m, 4)[0] == _lib.HMX_ERR_ARG
     assert nbytes(0, m, _lib.HMX_FIT_FP64)[0] == _lib.HMX_ERR_ARG
