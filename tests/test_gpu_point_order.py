@@ -7,8 +7,9 @@ called on 1024-point pieces (below it, so unsorted), for scattered points,
 a raster grid, a line, one repeated point and a cluster with outliers:
 every field, NaN at the same places, within 1e-13 x max|field| (the sorted
 waves may take other closed forms, whose roundings differ); and against the
-CPU oracle at sampled points (1e-10).  Also the layer accessor and
-prism_magnetic.
+CPU oracle at sampled points (1e-10).  Also the layer accessor,
+prism_magnetic, the smallest sorted size, a ragged point count and points
+with NaN coordinates.
 """
 
 import numpy as np
@@ -153,3 +154,22 @@ def test_sorted_ragged_point_counts(hm, model, n_obs):
     ref = np.concatenate(parts, axis=-1)
     for k, f in enumerate(TEN):
         assert_close(got[k], ref[k], 1e-13, err_msg=f"{n_obs} points, {f}")
+
+
+def test_sorted_with_nan_coordinates(hm, model):
+    """Points with a NaN coordinate go to the first cell of the sort and give
+    NaN at their own index; the other points are unaffected."""
+    prisms, density, _ = model
+    e, n, u = (np.array(c) for c in _points("scattered"))
+    e[[5, 900]] = np.nan
+    n[[17, 2048]] = np.nan
+    u[[33, 4095]] = np.nan
+    coords = (e, n, u)
+    got = _fields(hm, coords, prisms, density, ("g_z", "g_zz"))
+    ref = _pieces(lambda c: _fields(hm, c, prisms, density, ("g_z", "g_zz")), coords)
+    bad = np.zeros(N_OBS, dtype=bool)
+    bad[[5, 900, 17, 2048, 33, 4095]] = True
+    for k in range(2):
+        assert np.isnan(got[k][bad]).all()
+        assert np.isfinite(got[k][~bad]).all()
+        assert_close(got[k], ref[k], 1e-13)
