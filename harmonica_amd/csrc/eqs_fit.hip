@@ -442,7 +442,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void g
                                                       int64_t M, int64_t N, double *C, int64_t ldc,
                                                       double beta, double sign, double shift,
                                                       int64_t tiles_n, int64_t bsa, int64_t bsb,
-                                                      int64_t bsc) {
+                                                      int64_t bsc,
+                                                      const int64_t *__restrict__ dims) {
     __shared__ T lds[2][2][kKC * kPad];
     // problem blockIdx.z of a strided batch (element strides of A, B, C)
     A.X += (int64_t)blockIdx.z * bsa;
@@ -463,6 +464,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void g
         cb = blockIdx.x - rb * tiles_n;
     }
     const int64_t r0 = rb * kST, c0 = cb * kST;
+    if (dims) {
+        // Problem z uses only its first dims[2z] rows (the rest of its panels
+        // are zero rows) and its first dims[2z + 1] rows and columns of C (the
+        // rest is zero, plus the shift on the diagonal): the K loop stops at
+        // its own rows, and tiles wholly past its size run no K loop at all.
+        const int64_t kz = dims[2 * blockIdx.z], mz = dims[2 * blockIdx.z + 1];
+        const int64_t kr = (kz + kKC - 1) / kKC * kKC;
+        K = (r0 < mz && c0 < mz) ? (kr < K ? kr : K) : 0;
+    }
     const bool fast = A.vec && B.vec && r0 + kST <= M && c0 + kST <= N && K % kKC == 0;
     if (fast)
         tile_k_loop<T, true, UPPER>(A, B, K, r0, c0, lds, M, N, C, ldc, beta, sign, shift);
@@ -477,30 +487,32 @@ template <class T>
 int launch_gemm(const T *A, int64_t lda, const T *B, int64_t ldb, int64_t K, int64_t M,
                 int64_t N, double *C, int64_t ldc, double beta, double sign, double shift,
                 bool upper, hipStream_t s, int batch = 1, int64_t bsa = 0, int64_t bsb = 0,
-                int64_t bsc = 0) {
+                int64_t bsc = 0, const int64_t *dims = nullptr) {
     if (M <= 0 || N <= 0 || batch <= 0) return HMX_OK;
     const int64_t tm = (M + kST - 1) / kST, tn = (N + kST - 1) / kST;
     const Panel<T> pa = make_panel(A, lda, K, M), pb = make_panel(B, ldb, K, N);
     if (upper) {
         hipLaunchKernelGGL((gemm_tn_kernel<T, true>),
                            dim3((unsigned)(tn * (tn + 1) / 2), 1, (unsigned)batch), dim3(256), 0,
-                           s, pa, pb, K, M, N, C, ldc, beta, sign, shift, tn, bsa, bsb, bsc);
+                           s, pa, pb, K, M, N, C, ldc, beta, sign, shift, tn, bsa, bsb, bsc, dims);
         HMX_LAUNCHED("gemm_tn_kernel<upper>");
     } else {
         hipLaunchKernelGGL((gemm_tn_kernel<T, false>), dim3((unsigned)(tm * tn), 1, (unsigned)batch),
                            dim3(256), 0, s, pa, pb, K, M, N, C, ldc, beta, sign, shift, tn, bsa,
-                           bsb, bsc);
+                           bsb, bsc, dims);
         HMX_LAUNCHED("gemm_tn_kernel");
     }
     return HMX_OK;
 }
 
+// dims (device, 2 per problem, or NULL): the rows and the size each problem
+// of the batch really has (see gemm_tn_kernel).
 template <class T>
 int launch_syrk(const T *X, int64_t ldx, int64_t K, int64_t m, double *C, int64_t ldc, double beta,
                 double sign, double shift, hipStream_t s, int batch = 1, int64_t bsx = 0,
-                int64_t bsc = 0) {
+                int64_t bsc = 0, const int64_t *dims = nullptr) {
     return launch_gemm<T>(X, ldx, X, ldx, K, m, m, C, ldc, beta, sign, shift, true, s, batch, bsx,
-                          bsx, bsc);
+                          bsx, bsc, dims);
 }
 
 int launch_gemm_tn(const double *A, int64_t lda, const double *B, int64_t ldb, int64_t K,
@@ -1853,7 +1865,7 @@ __global__ void mfma_probe_kernel(int dtype, const double *__restrict__ a,
 struct BatchLayout {
     int64_t m_pad, n_max, nblk;
     size_t x_off, a_off, dinv_off, scale_off, vec_off, tmp_off, part_off, split_off, dinfo_off,
-        err_off, infos_off, total;
+        err_off, infos_off, dims_off, total;
 };
 
 inline size_t al256(size_t b) { return (b + 255) / 256 * 256; }
@@ -1887,6 +1899,8 @@ BatchLayout batch_layout(int64_t n_max, int64_t m_max, int64_t batch) {
     off += al256(solve_err_bytes());
     L.infos_off = off;
     off += al256(B * sizeof(int));
+    L.dims_off = off;  // per window: its data rows and its sources (int64)
+    off += al256(2 * B * sizeof(int64_t));
     L.total = off;
     return L;
 }
@@ -1945,10 +1959,13 @@ int eqs_factor_batch(const EqsWindow *wins, int64_t batch, int64_t n_max, int64_
                        batch * bs_x);
     HMX_LAUNCHED("fill_zero_f64_kernel");
     // column scales and the scaled jacobian of each window (zero padding)
+    std::vector<int64_t> dims(2 * (size_t)batch);
     for (int64_t b = 0; b < batch; ++b) {
         const EqsWindow &w = wins[b];
         if (w.nd <= 0 || w.m <= 0 || w.nd > n_max || w.m > m_max)
             return fail(HMX_ERR_ARG, "window %lld of the batch has a bad size", (long long)b);
+        dims[2 * b] = w.nd;
+        dims[2 * b + 1] = w.m;
         Fit f = batch_fit(w, damping, L, buf, b, s);
         int st = f.col_stats();
         if (st) return st;
@@ -1958,10 +1975,16 @@ int eqs_factor_batch(const EqsWindow *wins, int64_t batch, int64_t n_max, int64_
                            L.m_pad);
         HMX_LAUNCHED("scaled_jacobian_kernel");
     }
-    // A_b = X_b^T X_b + damping I for the whole batch, then the factors
+    // A_b = X_b^T X_b + damping I for the whole batch, then the factors;
+    // each window's product over its own rows and sources only (the padding
+    // is zero), the tiles past its sources left at zero plus the shift.
+    auto *d_dims = reinterpret_cast<int64_t *>(buf + L.dims_off);
+    HMX_HIP(hipMemcpyAsync(d_dims, dims.data(), dims.size() * sizeof(int64_t),
+                           hipMemcpyHostToDevice, s));
+    HMX_HIP(hipStreamSynchronize(s));  // dims is host memory of this frame
     double *A = reinterpret_cast<double *>(buf + L.a_off);
     int st = launch_syrk<double>(X, L.m_pad, L.n_max, L.m_pad, A, L.m_pad, 0.0, 1.0, damping, s,
-                                 (int)batch, bs_x, bs_a);
+                                 (int)batch, bs_x, bs_a, d_dims);
     if (st) return st;
     ev.record(1, s);
     Fit f = batch_fit(wins[0], damping, L, buf, 0, s);
