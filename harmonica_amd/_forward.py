@@ -168,6 +168,8 @@ def discard_null_prisms(prisms, density):
     west, east, south, north, bottom, top = tuple(prisms[:, i] for i in range(6))
     null_prisms = (west == east) | (south == north) | (bottom == top)
     null_prisms[density == 0] = True
+    if not null_prisms.any():  # nothing to drop: no copy of the arrays
+        return prisms, density
     keep = np.logical_not(null_prisms)
     return prisms[keep, :], density[keep]
 

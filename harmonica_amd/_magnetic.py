@@ -68,6 +68,8 @@ def _discard_null_prisms(prisms, magnetization):
     mag_e, mag_n, mag_u = magnetization
     null_mag = (mag_e == 0) & (mag_n == 0) & (mag_u == 0)
     null_prisms[null_mag] = True
+    if not null_prisms.any():  # nothing to drop: no copy of the arrays
+        return prisms, tuple(magnetization)
     keep = np.logical_not(null_prisms)
     return prisms[keep, :], tuple(m[keep] for m in magnetization)
 
