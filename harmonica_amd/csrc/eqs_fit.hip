@@ -1557,6 +1557,29 @@ struct Fit {
         return HMX_OK;
     }
 
+    // Two dot products x1.y1 (n1 terms) and x2.y2 (n2 terms) with one trip
+    // to the host: 128 partial sums each, added in order.
+    int dot2(const double *x1, const double *y1, int64_t n1, const double *x2, const double *y2,
+             int64_t n2, double *o1, double *o2) {
+        const int nb = 128;
+        hipLaunchKernelGGL(reduce_partial_kernel, dim3(nb), dim3(256), 0, s, 0, n1, (int64_t)1, x1,
+                           y1, part);
+        HMX_LAUNCHED("reduce_partial_kernel");
+        hipLaunchKernelGGL(reduce_partial_kernel, dim3(nb), dim3(256), 0, s, 0, n2, (int64_t)1, x2,
+                           y2, part + nb);
+        HMX_LAUNCHED("reduce_partial_kernel");
+        HMX_HIP(hipMemcpyAsync(host_part, part, 2 * nb * sizeof(double), hipMemcpyDeviceToHost, s));
+        HMX_HIP(hipStreamSynchronize(s));
+        double a1 = 0.0, a2 = 0.0;
+        for (int i = 0; i < nb; ++i) {
+            a1 += host_part[i];
+            a2 += host_part[nb + i];
+        }
+        *o1 = a1;
+        *o2 = a2;
+        return HMX_OK;
+    }
+
     int reduce(int mode, const double *x, const double *y, int64_t n, int64_t stride, double *out) {
         const int nb = 256;
         hipLaunchKernelGGL(reduce_partial_kernel, dim3(nb), dim3(256), 0, s, mode, n, stride, x, y, part);
@@ -1771,25 +1794,20 @@ struct Fit {
         if (st) return st;
         hipLaunchKernelGGL(sub_kernel, dim3(gm), dim3(256), 0, s, m, b, q, r);  // r = b - A x
         HMX_LAUNCHED("sub_kernel");
-        double ynorm = 0.0;
-        st = reduce(0, t_dat, t_dat, nd, 1, &ynorm);  // |J S x|^2
-        if (st) return st;
         hipLaunchKernelGGL(copy_kernel, dim3(gm), dim3(256), 0, s, m, r, wk);
         HMX_LAUNCHED("copy_kernel");
         st = solve(wk, wk2, z);
         if (st) return st;
         hipLaunchKernelGGL(copy_kernel, dim3(gm), dim3(256), 0, s, m, z, pd);
         HMX_LAUNCHED("copy_kernel");
-        double rz = 0.0;
-        st = reduce(0, r, z, m, 1, &rz);
+        double ynorm = 0.0, rz = 0.0;  // |J S x|^2 (t_dat is still J S x) and r.z
+        st = dot2(t_dat, t_dat, nd, r, z, m, &ynorm, &rz);
         if (st) return st;
         for (iters = 1; iters <= allowed; ++iters) {
             st = apply(pd, t_dat, q);  // q = A p, t = J S p
             if (st) return st;
             double pq = 0.0, tt = 0.0;
-            st = reduce(0, pd, q, m, 1, &pq);
-            if (st) return st;
-            st = reduce(0, t_dat, t_dat, nd, 1, &tt);
+            st = dot2(pd, q, m, t_dat, t_dat, nd, &pq, &tt);
             if (st) return st;
             if (!(pq > 0.0)) break;
             const double a = rz / pq;
