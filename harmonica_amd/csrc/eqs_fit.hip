@@ -728,13 +728,41 @@ __device__ __forceinline__ void wait_pair(const double *p0, bool v0, const doubl
     }
 }
 
+// Position of block (i, j), j > i, among the blocks above the diagonal of
+// an nb x nb block matrix, stored row by row.
+__host__ __device__ inline int64_t upper_block_index(int64_t i, int64_t j, int64_t nb) {
+    return i * (nb - 1) - i * (i - 1) / 2 + (j - i - 1);
+}
+
+// The factor's blocks above the diagonal in FP32, each kNB x kNB block
+// contiguous and row-major (zero past m), block (i, j) at
+// upper_block_index(i, j, nb) * kNB^2: the substitutions of the mixed fits,
+// where the factor is only a preconditioner, read these (half the bytes,
+// in streams of 64 KB) instead of its FP64 rows (1 KB runs).
+__global__ __launch_bounds__(256) void pack_upper_f32_kernel(const double *__restrict__ A,
+                                                             int64_t ld, int64_t m, int nb,
+                                                             float *__restrict__ Ub) {
+    const int j = blockIdx.x, i = blockIdx.y;
+    if (j <= i) return;
+    float *dst = Ub + upper_block_index(i, j, nb) * kNB * kNB;
+    const int c = threadIdx.x & (kNB - 1);
+    const int64_t col = (int64_t)j * kNB + c;
+    for (int rr = threadIdx.x >> 7; rr < kNB; rr += 2) {
+        const int64_t row = (int64_t)i * kNB + rr;
+        dst[rr * kNB + c] = (row < m && col < m) ? (float)A[row * ld + col] : 0.0f;
+    }
+}
+
 // Forward: y_b = Dinv_b^T (r_b - sum_{k < b} U_kb^T y_k), U_kb = rows of
 // block k, columns of block b.  Thread (g, t) = (tid / 128, tid % 128)
 // takes column t of the block and rows 16 g .. 16 g + 15 of every block k
-// (lanes 0-15 of its wavefront poll those 16 entries of y_k).
+// (lanes 0-15 of its wavefront poll those 16 entries of y_k).  kBlocked:
+// the blocks U_kb come from the FP32 copy Ub (pack_upper_f32_kernel).
+template <bool kBlocked>
 __global__ __launch_bounds__(1024) void trsv_fwd_sync_kernel(
-    const double *__restrict__ U, int64_t ld, int64_t m, const double *__restrict__ Dinv,
-    const double *__restrict__ r, double *__restrict__ y, int *__restrict__ err) {
+    const double *__restrict__ U, const float *__restrict__ Ub, int64_t ld, int64_t m,
+    const double *__restrict__ Dinv, const double *__restrict__ r, double *__restrict__ y,
+    int *__restrict__ err) {
     __shared__ double dinv[kNB * kNB];  // Dinv_b, copied during the dependency loop
     __shared__ double part[8][kNB];
     __shared__ double z[kNB];
@@ -746,12 +774,19 @@ __global__ __launch_bounds__(1024) void trsv_fwd_sync_kernel(
     // the right-hand side entry, loaded now rather than on the chain
     const double rc = (g == 0 && cv) ? r[c] : 0.0;
     double u[16];
-    // Columns past m (last block only) read column m - 1: finite values
-    // whose sums are dropped (z = 0 there).
+    // Columns past m (last block only) read column m - 1 (FP64) or the
+    // copy's zero padding: finite values whose sums are dropped (z = 0 there).
+    const int64_t nbk = (m + kNB - 1) / kNB;
     auto load_u = [&](int k) {
-        const double *col = U + ((int64_t)k * kNB + 16 * g) * ld + (cv ? c : m - 1);
+        if constexpr (kBlocked) {
+            const float *blk = Ub + upper_block_index(k, b, nbk) * kNB * kNB + (16 * g) * kNB + t;
 #pragma unroll
-        for (int q = 0; q < 16; ++q) u[q] = col[q * ld];
+            for (int q = 0; q < 16; ++q) u[q] = (double)blk[q * kNB];
+        } else {
+            const double *col = U + ((int64_t)k * kNB + 16 * g) * ld + (cv ? c : m - 1);
+#pragma unroll
+            for (int q = 0; q < 16; ++q) u[q] = col[q * ld];
+        }
     };
     double acc = 0.0;
     if (b > 0) load_u(0);
@@ -791,10 +826,13 @@ __global__ __launch_bounds__(1024) void trsv_fwd_sync_kernel(
 // Backward: x_b = Dinv_b (y_b - sum_{k > b} U_bk x_k).  Wavefront v of the
 // workgroup takes rows 8 v .. 8 v + 7 of the block, lane l columns l and
 // l + 64 of every block k (partial sums per lane, added across the lanes
-// once at the end); lane l polls entries l and l + 64 of x_k.
+// once at the end); lane l polls entries l and l + 64 of x_k.  kBlocked:
+// the blocks U_bk come from the FP32 copy Ub.
+template <bool kBlocked>
 __global__ __launch_bounds__(1024) void trsv_bwd_sync_kernel(
-    const double *__restrict__ U, int64_t ld, int64_t m, const double *__restrict__ DinvT,
-    const double *__restrict__ y, double *__restrict__ x, int *__restrict__ err) {
+    const double *__restrict__ U, const float *__restrict__ Ub, int64_t ld, int64_t m,
+    const double *__restrict__ DinvT, const double *__restrict__ y, double *__restrict__ x,
+    int *__restrict__ err) {
     __shared__ double dinvt[kNB * kNB];  // DinvT_b, copied during the dependency loop
     __shared__ double part[8][kNB];
     __shared__ double z[kNB];
@@ -813,15 +851,24 @@ __global__ __launch_bounds__(1024) void trsv_bwd_sync_kernel(
     // Rows and columns past m read row or column m - 1: finite values that
     // meet x = 0 (columns) or whose sums are dropped (rows, z = 0 there).
     auto load_u = [&](int k) {
-        const int64_t cb = (int64_t)k * kNB;
-        const int64_t c0 = cb + lane < m ? cb + lane : m - 1;
-        const int64_t c1 = cb + lane + 64 < m ? cb + lane + 64 : m - 1;
+        if constexpr (kBlocked) {  // blocks right of the diagonal: rows all below m
+            const float *blk = Ub + upper_block_index(b, k, nb) * kNB * kNB + (8 * v) * kNB + lane;
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            const int64_t row = r0 + 8 * v + i;
-            const double *p = U + (row < m ? row : m - 1) * ld;
-            u[2 * i] = p[c0];
-            u[2 * i + 1] = p[c1];
+            for (int i = 0; i < 8; ++i) {
+                u[2 * i] = (double)blk[i * kNB];
+                u[2 * i + 1] = (double)blk[i * kNB + 64];
+            }
+        } else {
+            const int64_t cb = (int64_t)k * kNB;
+            const int64_t c0 = cb + lane < m ? cb + lane : m - 1;
+            const int64_t c1 = cb + lane + 64 < m ? cb + lane + 64 : m - 1;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int64_t row = r0 + 8 * v + i;
+                const double *p = U + (row < m ? row : m - 1) * ld;
+                u[2 * i] = p[c0];
+                u[2 * i + 1] = p[c1];
+            }
         }
     };
     double acc[8];
@@ -1529,6 +1576,9 @@ struct Fit {
     // pointer once a solve has been set up
     int *err_word = nullptr, *terr = nullptr;
     double *split = nullptr;  // partial sums of the split reductions
+    // FP32 blocked copy of the factor's off-diagonal blocks, read by the
+    // substitutions when set (use_upper_f32)
+    const float *Ub = nullptr;
 
     // out = (1 / s) J^T W t + alpha p (W = diag(w_), w_ and p may be NULL).
     int jt(const double *w_, const double *t, double alpha_, const double *p, double *out) {
@@ -1755,14 +1805,44 @@ struct Fit {
         const int nb = (int)((n + kNB - 1) / kNB);
         hipLaunchKernelGGL(fill_sentinel_kernel, dim3(blocks_for(n)), dim3(256), 0, s, y, n);
         HMX_LAUNCHED("fill_sentinel_kernel");
-        hipLaunchKernelGGL(trsv_fwd_sync_kernel, dim3(nb), dim3(1024), 0, s, A, n, n, Dinv, r, y,
-                           terr);
+        if (Ub)
+            hipLaunchKernelGGL(trsv_fwd_sync_kernel<true>, dim3(nb), dim3(1024), 0, s, A, Ub, n, n,
+                               Dinv, r, y, terr);
+        else
+            hipLaunchKernelGGL(trsv_fwd_sync_kernel<false>, dim3(nb), dim3(1024), 0, s, A, Ub, n,
+                               n, Dinv, r, y, terr);
         HMX_LAUNCHED("trsv_fwd_sync_kernel");
         hipLaunchKernelGGL(fill_sentinel_kernel, dim3(blocks_for(n)), dim3(256), 0, s, x, n);
         HMX_LAUNCHED("fill_sentinel_kernel");
-        hipLaunchKernelGGL(trsv_bwd_sync_kernel, dim3(nb), dim3(1024), 0, s, A, n, n, DinvT, y, x,
-                           terr);
+        if (Ub)
+            hipLaunchKernelGGL(trsv_bwd_sync_kernel<true>, dim3(nb), dim3(1024), 0, s, A, Ub, n, n,
+                               DinvT, y, x, terr);
+        else
+            hipLaunchKernelGGL(trsv_bwd_sync_kernel<false>, dim3(nb), dim3(1024), 0, s, A, Ub, n,
+                               n, DinvT, y, x, terr);
         HMX_LAUNCHED("trsv_bwd_sync_kernel");
+        return HMX_OK;
+    }
+
+    // Bytes of the FP32 copy of the factor's blocks above the diagonal.
+    int64_t upper_f32_bytes() const {
+        const int64_t nb = (fm() + kNB - 1) / kNB;
+        return nb * (nb - 1) / 2 * kNB * kNB * (int64_t)sizeof(float);
+    }
+
+    // Packs that copy into `dst` (upper_f32_bytes() bytes) and has the
+    // substitutions read it from then on (the factor is then only a
+    // preconditioner: the FP32 rounding of U moves the preconditioned
+    // operator, not the solution CG converges to).
+    int use_upper_f32(float *dst) {
+        const int64_t n = fm();
+        const int nb = (int)((n + kNB - 1) / kNB);
+        if (nb > 1) {
+            hipLaunchKernelGGL(pack_upper_f32_kernel, dim3(nb, nb), dim3(256), 0, s, A, n, n, nb,
+                               dst);
+            HMX_LAUNCHED("pack_upper_f32_kernel");
+        }
+        Ub = dst;
         return HMX_OK;
     }
 
@@ -2375,6 +2455,14 @@ int hmx_eqs_fit(const double *easting, const double *northing, const double *upw
     // the caller asked for a direct solve.
     const int allowed = (tau > 0.0 && max_iter == 0) ? 200 : max_iter;
     if (!use_svd) {
+        // Mixed modes: the factor (of the low-precision normal matrix) is a
+        // preconditioner only, and the substitutions read an FP32 blocked
+        // copy of it, held where the saved normal matrix was (2 m^2 bytes
+        // at most, against its 8 m^2).
+        if (mixed && allowed > 0 && f.upper_f32_bytes() <= (int64_t)(m * m * sizeof(double))) {
+            st = f.use_upper_f32(reinterpret_cast<float *>(A_copy));
+            if (st) return st;
+        }
         st = f.solve_refine(b, x, r, z, pd, q, wk, wk2, allowed, rtol, iters, last_rel);
         if (st) return st;
         // an unregularised fit whose refinement did not converge

@@ -285,7 +285,9 @@ def test_split_normal_matrix_direct_solve(hm, precision, n_data, n_src):
 def test_mixed_fit_small_and_ragged(hm, precision, n_data, n_src):
     """Mixed fits where padding dominates: 7 data in one 64-row chunk (the
     rest zero digits), 65 and 1025 data one row past a chunk and a row
-    group, 5 to 130 sources in partial 128-column tiles.  Refined to the
+    group, 5 to 130 sources in partial 128-column tiles (with 130, the
+    substitutions read the FP32 blocked copy of the factor's one block
+    above the diagonal, 126 of its columns zero padding).  Refined to the
     FP64 normal equations, the coefficients match the FP64 fit's."""
     from harmonica_amd._eqs import fit_device
 
