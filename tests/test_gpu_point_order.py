@@ -7,9 +7,9 @@ called on 1024-point pieces (below it, so unsorted), for scattered points,
 a raster grid, a line, one repeated point and a cluster with outliers:
 every field, NaN at the same places, within 1e-13 x max|field| (the sorted
 waves may take other closed forms, whose roundings differ); and against the
-CPU oracle at sampled points (1e-10).  Also the layer accessor,
-prism_magnetic, the smallest sorted size, a ragged point count and points
-with NaN coordinates.
+CPU oracle at sampled points (1e-10).  Also the layer accessor (with and
+without NaN holes), prism_magnetic, the smallest sorted size, a ragged
+point count and points with NaN coordinates.
 """
 
 import numpy as np
@@ -120,6 +120,27 @@ def test_sorted_layer_accessor(hm):
     for field in ("g_z", "g_zz"):
         got = layer.prism_layer.gravity(coords, field=field)
         ref = _pieces(lambda c: layer.prism_layer.gravity(c, field=field), coords)
+        assert_close(got, ref, 1e-13, err_msg=field)
+
+
+def test_sorted_layer_with_holes(hm):
+    """A layer whose density has NaN holes (cells the forward model skips,
+    as the reference's layer accessor does) and a top surface with NaN
+    holes, through the sorted path: the same as 1024-point pieces."""
+    rng = np.random.default_rng(24)
+    easting, northing = np.linspace(0, 2e4, 512), np.linspace(0, 2e4, 512)
+    surface = rng.uniform(-500, 800, (512, 512))
+    density = rng.uniform(2000, 3000, (512, 512))
+    density[rng.random((512, 512)) < 0.1] = np.nan
+    surface[rng.random((512, 512)) < 0.05] = np.nan
+    layer = hm.prism_layer((easting, northing), surface, -1000.0, properties={"density": density})
+    coords = _points("grid")
+    for field in ("g_z", "g_ez"):
+        with pytest.warns(UserWarning, match="NaN values in 'density'"):
+            got = layer.prism_layer.gravity(coords, field=field, density_name="density")
+            ref = _pieces(
+                lambda c: layer.prism_layer.gravity(c, field=field, density_name="density"), coords)
+        assert np.isfinite(got).all()
         assert_close(got, ref, 1e-13, err_msg=field)
 
 
